@@ -1,0 +1,166 @@
+"""Benchmark: perft nodes/s on one MI355X.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--leaf 1|2] [--batch]
+
+Workload (one step): perft(7) of the config-2 position -- the start position
+after 8 seeded random plies (seed 2, SplitMix64 walk over canonically sorted
+legal moves, re-drawn if terminal), no transposition table, one plain BFS
+call of the whole depth on the GPU (BFS levels + leaf kernel).  13,069,312,759
+leaf nodes per step; every step's count is checked against the oracle's golden
+count.  Inputs live on the device for the whole timed region (the root position
+is 33 bytes; every BFS level is built in HBM).
+
+The JSON line carries:
+  roofline      the dominant kernel (the leaf) against the HBM peak, from
+                live HIP-event timing of every launch on the library stream
+                and the algorithmic bytes 6K + 33P per launch (K leaf entries,
+                P parent boards).  The leaf is VALU-bound integer work, so
+                its HBM fraction is small by nature; `roofline_levels` gives
+                the same for the BFS level kernels, which are HBM-bound.
+  cpu_baseline  the oracle (a C restatement of the reference's single-threaded
+                -cpu path) on this host, one core, on a bounded sample of the
+                same tree: perft(6) under the first root moves.
+With --batch it also times the 4096-position config-3 batch, batched vs plain.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def load_fixture(name):
+    with open(os.path.join(ROOT, "tests", "golden", name)) as f:
+        return [json.loads(line) for line in f if line.strip()]
+
+
+def cpu_baseline(fen: str, depth: int, budget_s: float = 15.0) -> dict:
+    """Time the oracle single-threaded on perft(depth-1) of successive root children."""
+    import oracle
+
+    root = oracle.parse_fen(fen)
+    moves = oracle.gen_moves(root)
+    nodes, t0, used = 0, time.perf_counter(), 0
+    for m in moves:
+        nodes += oracle.perft(oracle.make_move(root, m), depth - 1, threads=1)
+        used += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": nodes / dt, "unit": "nodes/s", "cores": 1, "kind": "port",
+            "sample": f"oracle perft({depth - 1}) under the first {used} of {len(moves)} root moves of the "
+                      f"config-2 position: {nodes} nodes in {dt:.1f} s, single-threaded like the reference -cpu path"}
+
+
+def roofline(stat: dict) -> dict:
+    avg_ms = stat["ms"] / stat["launches"]
+    bytes_per_launch = stat["bytes"] / stat["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "kernel": stat["name"], "avg_launch_ms": round(avg_ms, 4),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--leaf", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+              "(a position's tree would be split at the root across GPUs with one 8-byte sum per GPU).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import perft_gpu_2026_amd as pg
+
+    fx = load_fixture("c2.jsonl")[0]
+    depth, want = fx["depth"], fx["counts"][-1]
+    print(f"[bench] init; config-2 position {fx['fen']} depth {depth}", file=sys.stderr, flush=True)
+    pg.init(0)
+    if a.leaf:
+        pg.set_leaf_plies(a.leaf)
+    pos = pg.parse_fen(fx["fen"])
+    for i in range(a.warmup):
+        c = pg.perft_bfs(pos, depth)
+        assert c == want, (c, want)
+        print(f"[bench] warmup {i} ok", file=sys.stderr, flush=True)
+    pg.profile_reset()
+    pg.profile_enable(True)
+    pg.device_synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        c = pg.perft_bfs(pos, depth)
+        if c != want:
+            raise SystemExit(f"count mismatch at step {i}: {c} != {want}")
+    pg.device_synchronize()
+    t1 = time.perf_counter()
+    pg.profile_enable(False)
+    stats = pg.profile_get()
+    for k, v in stats.items():
+        v["name"] = k
+    dt = t1 - t0
+    value = want * a.steps / dt
+    leaf = stats.get("leaf2") or stats.get("leaf1")
+    levels = [v for k, v in stats.items() if k in ("makecount", "scan", "expand", "genmoves")]
+    top_level = max(levels, key=lambda v: v["ms"]) if levels else None
+    out = {
+        "metric": "perft nodes/s on 1 MI355X, batched vs plain BFS; p50/p99 query latency",
+        "value": value,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
+                   "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 1},
+        "roofline": roofline(leaf) if leaf else None,
+        "roofline_levels": roofline(top_level) if top_level else None,
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                        "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
+                    for k, v in stats.items()},
+    }
+    if a.batch:
+        rows = load_fixture("c3.jsonl")
+        ps = [pg.parse_fen(r["fen"]) for r in rows]
+        ds = [r["depth"] for r in rows]
+        total = sum(r["count"] for r in rows)
+        t0 = time.perf_counter()
+        got = pg.perft_batch_plain(ps, ds)
+        t_plain = time.perf_counter() - t0
+        assert got == [r["count"] for r in rows]
+        res = {"plain_nodes_per_s": total / t_plain, "plain_s": t_plain, "nodes": total}
+        try:
+            t0 = time.perf_counter()
+            got = pg.perft_batch(ps, ds)
+            t_b = time.perf_counter() - t0
+            assert got == [r["count"] for r in rows]
+            res.update({"batched_nodes_per_s": total / t_b, "batched_s": t_b, "ratio": t_plain / t_b})
+        except pg.PerftError as e:
+            res["batched_error"] = str(e)
+        out["batch_config3"] = res
+    if not a.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(fx["fen"], depth)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

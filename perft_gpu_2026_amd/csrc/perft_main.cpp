@@ -1,0 +1,154 @@
+// perft_main.cpp -- the `perft_gpu` command line, a drop-in for perft.cu:13-186:
+//   perft_gpu <fen> <depth> [<launchdepth>] [-nott] [-dtt <MB>] [-htt <MB>] [-ld <N>] [-cpu]
+// Flags may appear anywhere; a positional third argument is the launch depth;
+// with no arguments it runs the start position to depth 10.  Output lines keep
+// the reference's form: "Perft(%02d): <count>, time: <s> seconds, nps: <n>".
+// Extra flags of this engine: -leaf <1|2> (leaf plies of the GPU BFS),
+// -arena <MB> (BFS arena size).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+
+#include "../../include/perft_gpu.h"
+
+extern "C" int pg_u128_to_string(uint64_t lo, uint64_t hi, char *buf, int n);
+extern "C" void pg_set_leaf_plies(int plies);
+
+static double now()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static void printResult(int depth, uint64_t lo, uint64_t hi, double secs)
+{
+    char buf[64];
+    pg_u128_to_string(lo, hi, buf, sizeof buf);
+    printf("\nPerft(%02d): %s, time: %g seconds", depth, buf, secs);
+    if (secs > 0) printf(", nps: %llu", (unsigned long long)(((double)hi * 18446744073709551616.0 + (double)lo) / secs));
+    printf("\n");
+    fflush(stdout);
+}
+
+int main(int argc, char *argv[])
+{
+    bool cpuMode = false, useTT = true;
+    long ldOverride = 0, dttMB = 0, httMB = 0, arenaMB = 0, leaf = 0;
+    // flags anywhere (perft.cu:17-57): consume them and compact argv
+    for (int i = 1; i < argc; i++) {
+        long *valueFlag = nullptr;
+        if (!strcmp(argv[i], "-cpu")) cpuMode = true;
+        else if (!strcmp(argv[i], "-nott")) useTT = false;
+        else if (!strcmp(argv[i], "-ld")) valueFlag = &ldOverride;
+        else if (!strcmp(argv[i], "-dtt")) valueFlag = &dttMB;
+        else if (!strcmp(argv[i], "-htt")) valueFlag = &httMB;
+        else if (!strcmp(argv[i], "-arena")) valueFlag = &arenaMB;
+        else if (!strcmp(argv[i], "-leaf")) valueFlag = &leaf;
+        else continue;
+        int take = 1;
+        if (valueFlag) {
+            if (i + 1 >= argc) break;
+            *valueFlag = atol(argv[i + 1]);
+            take = 2;
+        }
+        for (int j = i; j + take < argc; j++) argv[j] = argv[j + take];
+        argc -= take;
+        i--;
+    }
+
+    if (!cpuMode) {
+        int s = pg_init(0, (uint64_t)arenaMB << 20);
+        if (s == PG_ERR_NO_DEVICE) {
+            printf("No of GPUs detected: 0\n");
+            printf("No GPUs found. Exiting.\n");
+            return 1;
+        }
+        printf("No of GPUs detected: 1\n");
+        if (s != PG_OK) {
+            printf("GPU init failed: %s\n", pg_last_error());
+            return 1;
+        }
+        printf("BFS arena: %.1f GiB\n", pg_arena_bytes() / 1073741824.0);
+    }
+    if (leaf == 1 || leaf == 2) pg_set_leaf_plies((int)leaf);
+
+    const char *fen = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1";
+    int maxDepth = 10;
+    if (argc >= 3) {
+        fen = argv[1];
+        maxDepth = atoi(argv[2]);
+    } else {
+        printf("\nUsage: perft_gpu <fen> <depth> [<launchdepth>] [-nott] [-dtt <MB>] [-htt <MB>] [-ld <N>] [-cpu]\n");
+        printf("\nAs no parameters were provided... running default test\n");
+    }
+    pg_position pos;
+    if (pg_parse_fen(fen, &pos) != PG_OK) {
+        printf("invalid FEN: %s\n", fen);
+        return 1;
+    }
+    if (maxDepth < 0 || maxDepth > 20) {
+        printf("depth %d out of range\n", maxDepth);
+        return 1;
+    }
+
+    if (cpuMode) {
+        printf("CPU mode\n");
+        fflush(stdout);
+        if (useTT) pg_tt_init(0, httMB ? (uint64_t)httMB : 1024);
+        for (int d = 1; d <= maxDepth; d++) {
+            uint64_t c = 0;
+            double t0 = now();
+            pg_perft_cpu(&pos, d, useTT, &c);
+            printResult(d, c, 0, now() - t0);
+        }
+        pg_tt_free();
+        return 0;
+    }
+
+    float bf = 30.0f;
+    int ld = 0;
+    pg_estimate_launch_depth(&pos, &bf, &ld);
+    // launch-depth policy, perft.cu:119-154
+    if (ld > 11) ld = 11;
+    if (ld < 6) ld = 6;
+    if (maxDepth >= 10 && ld < 9) ld = 9;
+    if (ldOverride > 0) ld = (int)ldOverride;
+    else if (argc >= 4) ld = atoi(argv[3]);
+    if (maxDepth < ld) ld = maxDepth;
+    int currentLD = ld;
+    printf("Launch depth: %d\n", currentLD);
+    fflush(stdout);
+    if (useTT) {
+        uint64_t dMB = dttMB ? (uint64_t)dttMB : 16384, hMB = httMB ? (uint64_t)httMB : 4096;
+        if (pg_tt_init(dMB, hMB) != PG_OK) printf("warning: TT init failed: %s\n", pg_last_error());
+    }
+    int rc = 0;
+    for (int d = 1; d <= maxDepth; d++) {
+        int eff = currentLD < d ? currentLD : d;
+        pg_u128 r{0, 0};
+        uint64_t oomBefore = pg_get_oom_fallback_count();
+        double t0 = now();
+        int s = pg_perft(&pos, d, eff, useTT, &r);
+        double t1 = now();
+        if (s != PG_OK) {
+            printf("perft(%d) failed: %s\n", d, pg_last_error());
+            rc = 1;
+            break;
+        }
+        printResult(d, r.lo, r.hi, t1 - t0);
+        uint64_t ooms = pg_get_oom_fallback_count() - oomBefore;
+        if (ooms > 0 && d >= ld) {
+            int newLD = pg_get_effective_ld();
+            if (newLD != currentLD) {
+                printf("  >> Launch depth adjusted: %d -> %d (OOM fallbacks: %llu)\n", currentLD, newLD,
+                       (unsigned long long)ooms);
+                currentLD = newLD;
+            }
+        }
+    }
+    pg_tt_free();
+    pg_shutdown();
+    return rc;
+}

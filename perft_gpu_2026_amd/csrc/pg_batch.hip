@@ -1,0 +1,653 @@
+// pg_batch.hip -- batched perft: thousands of positions of uneven size in one
+// device-resident pass (no reference counterpart; the reference perfts one
+// position per call, perft_kernels.cu:620).
+//
+// Design:
+//  * every root is normalised to white-to-move by a colour flip (vertical
+//    mirror + colour swap, perft-invariant), so a BFS level has one side to
+//    move and the kernels stay colour-templated;
+//  * roots are grouped by depth; a group is one multi-root BFS: level 0 = the
+//    roots, each level's entries carry (parent, move, root id), children keep
+//    their parents' order so a root's entries stay contiguous;
+//  * level sizes never leave the device: after each scan a one-thread kernel
+//    bump-allocates the next level inside the arena (setting an OOM flag when
+//    it does not fit) and every kernel runs a fixed persistent grid that
+//    reads its size from the level descriptor.  With fixed launch shapes the
+//    whole level sequence of a group is captured once in a hipGraph and
+//    replayed for every batch of the same shape;
+//  * the leaf is persistent: waves pull 32-entry chunks from a device work
+//    queue (an atomic cursor), so roots with huge subtrees and roots with tiny
+//    ones share the GPU without a tail; results go to per-root u64 counters
+//    with one atomic per wave and root;
+//  * an autotuner picks the leaf plies (1 or 2) per (depth, group size) by
+//    timing both on a sample of the group and caches the choice.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <vector>
+
+#include "pg_device.h"
+#include "pg_launcher.h"
+#include "pg_tt.h"
+
+namespace pg {
+
+constexpr int MAXLV = 24;
+constexpr int B_CHUNK = 32;  // leaf entries per work-queue pop
+
+struct LvlDev {
+    u64 n;                 // entries (level 0: roots)
+    u32 *idx;              // parent index into the previous level (level >= 1)
+    u16 *mv;               // move applied to the parent (level >= 1)
+    u32 *root;             // root id (level >= 1)
+    QuadBitBoard *boards;  // positions of this level (level 0: the roots)
+    u8 *states;
+    u32 *cnt;              // child counts, then their inclusive prefix
+    u32 *tileSums;
+    u64 *tileOffs;
+    u32 *parts;            // expand partitions of the next level
+};
+
+struct BatchDev {
+    char *arena;
+    u64 arenaCap, arenaOff;  // device-side bump allocator over the arena
+    u32 oom, leafLevel;
+    u64 work;                // leaf work-queue cursor
+    u64 *rootCounts;
+    LvlDev lv[MAXLV];
+};
+
+__device__ __forceinline__ void *dalloc(BatchDev *d, u64 bytes)
+{
+    bytes = (bytes + 255) & ~(u64)255;
+    if (d->arenaOff + bytes > d->arenaCap) {
+        d->oom = 1;
+        return nullptr;
+    }
+    void *p = d->arena + d->arenaOff;
+    d->arenaOff += bytes;
+    return p;
+}
+
+// ---------------------------------------------------------------- kernels
+// zero the per-root counters and the work cursor (a kernel, not a memset
+// node, so the captured graph replays correctly)
+__global__ void k_b_init(BatchDev *d)
+{
+    const u64 nRoots = d->lv[0].n;
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < nRoots; i += (u64)gridDim.x * blockDim.x) d->rootCounts[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) d->work = 0;
+}
+
+__global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
+{
+    if (d->oom) return;
+    LvlDev &a = d->lv[0];
+    u64 tiles = (a.n + BS - 1) / BS;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u64 i = t * BS + threadIdx.x;
+        u32 c = 0;
+        if (i < a.n) {
+            c = countMoves<WHITE>(a.boards[i], a.states[i]);
+            a.cnt[i] = c;
+        }
+        u64 s = blockSum64(c);
+        if (threadIdx.x == 0) a.tileSums[t] = (u32)s;
+        __syncthreads();
+    }
+}
+
+template <int C>  // side to move in the parent boards
+__global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    LvlDev &a = d->lv[L];
+    const LvlDev &p = d->lv[L - 1];
+    u64 tiles = (a.n + BS - 1) / BS;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u64 i = t * BS + threadIdx.x;
+        u32 c = 0;
+        if (i < a.n) {
+            u32 pi = a.idx[i];
+            QuadBitBoard q = p.boards[pi];
+            u8 st = p.states[pi];
+            makeMove<C>(q, st, a.mv[i]);
+            a.boards[i] = q;
+            a.states[i] = st;
+            c = countMoves<1 - C>(q, st);
+            a.cnt[i] = c;
+        }
+        u64 s = blockSum64(c);
+        if (threadIdx.x == 0) a.tileSums[t] = (u32)s;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_b_scan_top(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    LvlDev &a = d->lv[L];
+    u64 tiles = (a.n + BS - 1) / BS;
+    __shared__ u64 wtot[16];
+    __shared__ u64 carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (u64 base = 0; base < tiles; base += 1024) {
+        u64 i = base + threadIdx.x;
+        u64 v = i < tiles ? a.tileSums[i] : 0;
+        u64 incl = waveInclScan64(v);
+        if (lane == 63) wtot[w] = incl;
+        __syncthreads();
+        u64 before = carry;
+        for (int k = 0; k < w; k++) before += wtot[k];
+        if (i < tiles) a.tileOffs[i] = before + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.tileOffs[tiles] = carry;
+}
+
+__global__ __launch_bounds__(BS) void k_b_scan_apply(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    LvlDev &a = d->lv[L];
+    u64 tiles = (a.n + BS - 1) / BS;
+    __shared__ u32 wtot[BS / 64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u64 i = t * BS + threadIdx.x;
+        u32 v = i < a.n ? a.cnt[i] : 0;
+        u32 incl = waveInclScan(v);
+        if (lane == 63) wtot[w] = incl;
+        __syncthreads();
+        u32 before = 0;
+        for (int k = 0; k < w; k++) before += wtot[k];
+        if (i < a.n) a.cnt[i] = (u32)(a.tileOffs[t] + before + incl);
+        __syncthreads();
+    }
+}
+
+// one thread: size the next level and carve its arrays out of the arena
+__global__ void k_b_alloc(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    LvlDev &a = d->lv[L];
+    LvlDev &b = d->lv[L + 1];
+    u64 tiles = (a.n + BS - 1) / BS;
+    u64 c = a.n ? a.tileOffs[tiles] : 0;
+    b.n = 0;
+    if (c >= 0xFFFFFFFFull) {
+        d->oom = 1;
+        return;
+    }
+    a.parts = (u32 *)dalloc(d, 4 * (c / EXPAND_TILE + 2));
+    b.idx = (u32 *)dalloc(d, 4 * c + 4);
+    b.mv = (u16 *)dalloc(d, 2 * c + 2);
+    b.root = (u32 *)dalloc(d, 4 * c + 4);
+    if ((u32)(L + 1) < d->leafLevel) {
+        u64 bt = (c + BS - 1) / BS;
+        b.boards = (QuadBitBoard *)dalloc(d, 32 * c + 32);
+        b.states = (u8 *)dalloc(d, c + 1);
+        b.cnt = (u32 *)dalloc(d, 4 * c + 4);
+        b.tileSums = (u32 *)dalloc(d, 4 * bt + 4);
+        b.tileOffs = (u64 *)dalloc(d, 8 * (bt + 1));
+    }
+    if (!d->oom) b.n = c;
+}
+
+__global__ __launch_bounds__(BS) void k_b_partitions(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev &a = d->lv[L];
+    u64 n = a.n, c = d->lv[L + 1].n;
+    if (!c) return;
+    u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
+    for (u64 t = (u64)blockIdx.x * BS + threadIdx.x; t <= tiles; t += (u64)gridDim.x * BS) {
+        u64 child = t * EXPAND_TILE;
+        if (child > c) child = c;
+        u64 p = upperBound(a.cnt, n, child);
+        a.parts[t] = (u32)(p < n ? p : n - 1);
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_b_expand(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev &a = d->lv[L];
+    LvlDev &b = d->lv[L + 1];
+    u64 c = b.n;
+    u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
+    __shared__ u32 win[EXPAND_WCAP];
+    const u32 *incl = a.cnt;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u64 c0 = t * EXPAND_TILE;
+        u64 p0 = a.parts[t], p1 = a.parts[t + 1];
+        u64 w = p1 - p0 + 1;
+        bool staged = w <= EXPAND_WCAP;
+        if (staged)
+            for (u64 k = threadIdx.x; k < w; k += BS) win[k] = incl[p0 + k];
+        __syncthreads();
+        for (int k = 0; k < EXPAND_TILE / BS; k++) {
+            u64 ch = c0 + (u64)k * BS + threadIdx.x;
+            if (ch >= c) break;
+            u64 lo = 0, hi = w;
+            while (lo < hi) {
+                u64 mid = (lo + hi) >> 1;
+                u32 v = staged ? win[mid] : incl[p0 + mid];
+                if ((u64)v <= ch) lo = mid + 1;
+                else hi = mid;
+            }
+            u32 parent = (u32)(p0 + lo);
+            b.idx[ch] = parent;
+            b.root[ch] = L == 0 ? parent : a.root[parent];
+        }
+        __syncthreads();
+    }
+}
+
+template <int C>  // side to move in this level's boards
+__global__ __launch_bounds__(BS) void k_b_genmoves(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev &a = d->lv[L];
+    u16 *out = d->lv[L + 1].mv;
+    u64 n = a.n;
+    const u32 *incl = a.cnt;
+    __shared__ u16 buf[GEN_LCAP];
+    u64 tiles = (n + BS - 1) / BS;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u64 b0 = t * BS, i = b0 + threadIdx.x;
+        u64 bEnd = b0 + BS < n ? b0 + BS : n;
+        u64 blockStart = b0 ? incl[b0 - 1] : 0;
+        u64 total = (u64)incl[bEnd - 1] - blockStart;
+        u64 excl = (i < n && i) ? (u64)incl[i - 1] : 0;
+        if (total <= GEN_LCAP) {
+            if (i < n) {
+                u16 *dst = buf + (excl - blockStart);
+                generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) { dst[k] = m; });
+            }
+            __syncthreads();
+            u16 *o = out + blockStart;
+            for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
+        } else if (i < n) {
+            u16 *dst = out + excl;
+            generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) { dst[k] = m; });
+        }
+        __syncthreads();
+    }
+}
+
+// add a per-lane value to its root's counter: one atomic per wave when the
+// lanes share a root (entries are root-ordered), else one per lane
+__device__ __forceinline__ void addToRoots(u64 *rootCounts, bool valid, u32 root, u64 v)
+{
+    u32 r0 = __shfl(root, __builtin_ctzll(__ballot(valid) | (1ULL << 63)));
+    bool same = valid ? root == r0 : true;
+    if (__all(same)) {
+        u64 s = waveSum64(valid ? v : 0);
+        if ((threadIdx.x & 63) == 0 && s) atomicAdd((unsigned long long *)&rootCounts[r0], (unsigned long long)s);
+    } else if (valid && v) {
+        atomicAdd((unsigned long long *)&rootCounts[root], (unsigned long long)v);
+    }
+}
+
+// persistent leaf, one ply: pop 64-entry chunks from the work queue
+template <int C>
+__global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
+{
+    if (d->oom) return;
+    const u32 L = d->leafLevel;
+    const LvlDev &a = d->lv[L];
+    const LvlDev &p = d->lv[L - 1];
+    const u64 n = a.n;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        u64 base = 0;
+        if (lane == 0) base = atomicAdd((unsigned long long *)&d->work, 64ull);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        u64 e = base + lane;
+        bool valid = e < n;
+        u32 v = 0, root = 0;
+        if (valid) {
+            u32 pi = a.idx[e];
+            QuadBitBoard q = p.boards[pi];
+            u8 st = p.states[pi];
+            makeMove<C>(q, st, a.mv[e]);
+            v = countMoves<1 - C>(q, st);
+            root = a.root[e];
+        }
+        addToRoots(d->rootCounts, valid, root, v);
+    }
+}
+
+// persistent fused two-ply leaf: the wave-cooperative scheme of k_leaf2
+// (child boards and the flat pair list in LDS) with work-queue chunks and
+// per-entry partial sums kept in LDS until the chunk is done
+constexpr int BL2_CAP = 1536;
+constexpr int BL2_WAVES = 4;
+template <int C>
+__global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
+{
+    if (d->oom) return;
+    __shared__ QuadBitBoard sb[BL2_WAVES][B_CHUNK];
+    __shared__ u8 ss[BL2_WAVES][B_CHUNK];
+    __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
+    __shared__ u32 sl[BL2_WAVES][BL2_CAP];
+    const u32 L = d->leafLevel;
+    const LvlDev &a = d->lv[L];
+    const LvlDev &p = d->lv[L - 1];
+    const u64 n = a.n;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (;;) {
+        u64 base = 0;
+        if (lane == 0) base = atomicAdd((unsigned long long *)&d->work, (unsigned long long)B_CHUNK);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        u32 cnt = 0, root = 0;
+        u64 e = base + lane;
+        bool valid = lane < B_CHUNK && e < n;
+        if (valid) {
+            u32 pi = a.idx[e];
+            QuadBitBoard q = p.boards[pi];
+            u8 st = p.states[pi];
+            makeMove<C>(q, st, a.mv[e]);
+            sb[w][lane] = q;
+            ss[w][lane] = st;
+            cnt = countMoves<1 - C>(q, st);
+            root = a.root[e];
+        }
+        if (lane < B_CHUNK) sacc[w][lane] = 0;
+        u32 incl = waveInclScan(cnt);
+        u32 off = incl - cnt;
+        u32 total = __shfl(incl, 63);
+        waveLdsSync();
+        for (u32 wb = 0; wb < total; wb += BL2_CAP) {
+            if (cnt && off < wb + BL2_CAP && off + cnt > wb) {
+                QuadBitBoard q = sb[w][lane];
+                u8 st = ss[w][lane];
+                u32 *dst = sl[w];
+                const u32 tag = (u32)lane << 16;
+                generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+                    u32 j = off + k - wb;
+                    if (j < BL2_CAP) dst[j] = (u32)m | tag;
+                });
+            }
+            waveLdsSync();
+            u32 wn = total - wb < (u32)BL2_CAP ? total - wb : (u32)BL2_CAP;
+            for (u32 j = lane; j < wn; j += 64) {
+                u32 x = sl[w][j];
+                u32 src = x >> 16;
+                QuadBitBoard c = sb[w][src];
+                u8 s2 = ss[w][src];
+                makeMove<1 - C>(c, s2, (u16)x);
+                atomicAdd(&sacc[w][src], countMoves<C>(c, s2));  // LDS atomic
+            }
+            waveLdsSync();
+        }
+        u64 v = valid ? sacc[w][lane] : 0;
+        addToRoots(d->rootCounts, valid, root, v);
+        waveLdsSync();
+    }
+}
+
+// ------------------------------------------------------------------ host
+struct GroupPlan {
+    int depth, plies, leafLevel;
+};
+
+struct BatchState {
+    BatchDev *dev = nullptr;  // device descriptor
+    BatchDev host{};          // host copy used to initialise it
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (depth, plies) -> graph
+    std::map<std::pair<int, int>, int> tuned;                // (depth, size bucket) -> plies
+};
+static BatchState g_batch;
+
+static unsigned persistentBlocks()
+{
+    int cu = lib().cuCount ? lib().cuCount : 256;
+    return (unsigned)(cu * 8);
+}
+
+// enqueue the whole level sequence of one group (capturable: fixed shapes,
+// sizes read on the device)
+static int enqueueGroup(const GroupPlan &g, hipStream_t st)
+{
+    BatchDev *d = g_batch.dev;
+    unsigned G = persistentBlocks();
+    hipLaunchKernelGGL(k_b_init, dim3(64), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_b_count0, dim3(G), dim3(BS), 0, st, d);
+    for (int L = 0; L < g.leafLevel; L++) {
+        int side = L & 1;  // level-L boards: white to move on even levels
+        if (L > 0) {
+            if (side == BLACK) hipLaunchKernelGGL(k_b_makecount<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
+            else hipLaunchKernelGGL(k_b_makecount<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+        }
+        hipLaunchKernelGGL(k_b_scan_top, dim3(1), dim3(1024), 0, st, d, L);
+        hipLaunchKernelGGL(k_b_scan_apply, dim3(G), dim3(BS), 0, st, d, L);
+        hipLaunchKernelGGL(k_b_alloc, dim3(1), dim3(1), 0, st, d, L);
+        hipLaunchKernelGGL(k_b_partitions, dim3(G), dim3(BS), 0, st, d, L);
+        hipLaunchKernelGGL(k_b_expand, dim3(G), dim3(BS), 0, st, d, L);
+        if (side == WHITE) hipLaunchKernelGGL(k_b_genmoves<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
+        else hipLaunchKernelGGL(k_b_genmoves<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+    }
+    int pside = (g.leafLevel - 1) & 1;  // side to move in the leaf entries' parents
+    if (g.plies == 1) {
+        if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1<WHITE>, dim3(G), dim3(BS), 0, st, d);
+        else hipLaunchKernelGGL(k_b_leaf1<BLACK>, dim3(G), dim3(BS), 0, st, d);
+    } else {
+        if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2<WHITE>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+        else hipLaunchKernelGGL(k_b_leaf2<BLACK>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+    }
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+static void flipColors(QuadBitBoard &q, u8 &state)
+{
+    u64 occ = q.bb[1] | q.bb[2] | q.bb[3];
+    u64 white = occ & ~q.bb[0];
+    q.bb[0] = __builtin_bswap64(white);
+    q.bb[1] = __builtin_bswap64(q.bb[1]);
+    q.bb[2] = __builtin_bswap64(q.bb[2]);
+    q.bb[3] = __builtin_bswap64(q.bb[3]);
+    state = (u8)((state & 0xF0) | ((state & 3) << 2) | ((state >> 2) & 3));
+}
+
+// run one group (roots already white to move) on the arena; OOM splits it
+static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *states, u64 nRoots, bool useGraph,
+                    u64 *counts)
+{
+    Lib &L = lib();
+    hipStream_t st = L.stream;
+    // arena layout: [roots | states | level-0 arrays | rootCounts | device bump region]
+    Bump A(L.arena, usableArena());
+    QuadBitBoard *dRoots = A.alloc<QuadBitBoard>(nRoots);
+    u8 *dStates = A.alloc<u8>(nRoots);
+    u32 *cnt0 = A.alloc<u32>(nRoots);
+    u64 tiles0 = (nRoots + BS - 1) / BS;
+    u32 *ts0 = A.alloc<u32>(tiles0);
+    u64 *to0 = A.alloc<u64>(tiles0 + 1);
+    u64 *rc = A.alloc<u64>(nRoots);
+    if (!dRoots || !dStates || !cnt0 || !ts0 || !to0 || !rc) return PG_ERR_ARENA_OOM;
+    PG_HIP(hipMemcpyAsync(dRoots, roots, nRoots * sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(dStates, states, nRoots, hipMemcpyHostToDevice, st));
+    BatchDev &h = g_batch.host;
+    memset(&h, 0, sizeof h);
+    h.arena = (char *)L.arena + A.off;
+    h.arenaCap = usableArena() - A.off;
+    h.leafLevel = (u32)g.leafLevel;
+    h.rootCounts = rc;
+    h.lv[0].n = nRoots;
+    h.lv[0].boards = dRoots;
+    h.lv[0].states = dStates;
+    h.lv[0].cnt = cnt0;
+    h.lv[0].tileSums = ts0;
+    h.lv[0].tileOffs = to0;
+    PG_HIP(hipMemcpyAsync(g_batch.dev, &h, sizeof h, hipMemcpyHostToDevice, st));
+    Profiler &P = L.prof;
+    P.begin(K_BATCH_GROUP, st);
+    if (useGraph) {
+        auto key = std::make_pair(g.depth, g.plies);
+        auto it = g_batch.graphs.find(key);
+        if (it == g_batch.graphs.end()) {
+            hipGraph_t graph;
+            PG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int s = enqueueGroup(g, st);
+            PG_HIP(hipStreamEndCapture(st, &graph));
+            if (s != PG_OK) return s;
+            hipGraphExec_t exec;
+            PG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            it = g_batch.graphs.emplace(key, exec).first;
+        }
+        PG_HIP(hipGraphLaunch(it->second, st));
+    } else {
+        PG_TRY(enqueueGroup(g, st));
+    }
+    P.end(0, (double)nRoots, st);
+    // one readback for the whole group: the OOM flag, then the counts
+    PG_HIP(hipMemcpyAsync(L.hostPinned, &g_batch.dev->oom, sizeof(u32), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipMemcpyAsync(counts, rc, nRoots * sizeof(u64), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    P.collect();
+    if (*(u32 *)L.hostPinned) return PG_ERR_ARENA_OOM;
+    return PG_OK;
+}
+
+static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8 *states, u64 nRoots, bool useGraph,
+                         u64 *counts)
+{
+    int s = runGroup(g, roots, states, nRoots, useGraph, counts);
+    if (s != PG_ERR_ARENA_OOM) return s;
+    if (nRoots == 1) {
+        // a single root too large for one pass: its children become a batch
+        // one ply shallower (each flipped back to white to move)
+        if (g.depth <= 2) return PG_ERR_ARENA_OOM;
+        u16 mv[MAX_MOVES];
+        u32 nc = generateMoves<WHITE>(roots[0], states[0], mv);
+        std::vector<QuadBitBoard> kids(nc);
+        std::vector<u8> ks(nc);
+        for (u32 i = 0; i < nc; i++) {
+            kids[i] = roots[0];
+            ks[i] = states[0];
+            makeMove<WHITE>(kids[i], ks[i], mv[i]);
+            flipColors(kids[i], ks[i]);
+        }
+        int plies = g.plies < g.depth - 1 ? g.plies : g.depth - 2;
+        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies};
+        std::vector<u64> kc(nc);
+        if (nc) PG_TRY(runGroupSplit(sub, kids.data(), ks.data(), nc, useGraph, kc.data()));
+        u64 total = 0;
+        for (u64 v : kc) total += v;
+        counts[0] = total;
+        return PG_OK;
+    }
+    u64 h = nRoots / 2;
+    PG_TRY(runGroupSplit(g, roots, states, h, useGraph, counts));
+    return runGroupSplit(g, roots + h, states + h, nRoots - h, useGraph, counts + h);
+}
+
+static int chooseLeafPlies(int depth, const std::vector<QuadBitBoard> &roots, const std::vector<u8> &states,
+                           bool useGraph)
+{
+    int bucket = 0;
+    for (u64 n = roots.size(); n > 1; n >>= 1) bucket++;
+    auto key = std::make_pair(depth, bucket);
+    auto it = g_batch.tuned.find(key);
+    if (it != g_batch.tuned.end()) return it->second;
+    // time both leaf forms on a sample of the group (second run of each, so
+    // graph capture and first-launch costs are excluded)
+    u64 ns = std::min<u64>(roots.size(), 256);
+    std::vector<u64> tmp(ns);
+    double best = 1e30;
+    int bestP = 2;
+    for (int p = 1; p <= 2 && p < depth; p++) {
+        GroupPlan g{depth, p, depth - p};
+        double t = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            auto t0 = std::chrono::steady_clock::now();
+            if (runGroupSplit(g, roots.data(), states.data(), ns, useGraph, tmp.data()) != PG_OK) {
+                t = 1e30;
+                break;
+            }
+            t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        if (t < best) best = t, bestP = p;
+    }
+    g_batch.tuned[key] = bestP;
+    return bestP;
+}
+
+int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_batch_options *opt, uint64_t *counts)
+{
+    Lib &L = lib();
+    if (!L.init) {
+        setError("pg_init has not run");
+        return PG_ERR_NOT_INIT;
+    }
+    std::lock_guard<std::mutex> lock(L.mu);
+    if (!g_batch.dev) PG_HIP(hipMalloc((void **)&g_batch.dev, sizeof(BatchDev)));
+    bool useGraph = !opt || opt->useGraph;
+    int forcedPlies = opt ? opt->leafPlies : 0;
+    // group by depth; depth <= 1 on the host as the reference does (perft_kernels.cu:623-626)
+    std::map<int, std::vector<int>> groups;
+    for (int i = 0; i < n; i++) {
+        int d = depths[i];
+        if (d > MAXLV - 2) {
+            setError("depth %d too large for the batched path", d);
+            return PG_ERR_INVALID;
+        }
+        if (d <= 1) {
+            QuadBitBoard q;
+            memcpy(q.bb, pos[i].bb, 32);
+            counts[i] = d == 0 ? 1 : countMoves(q, pos[i].state, pos[i].color);
+        } else {
+            groups[d].push_back(i);
+        }
+    }
+    for (auto &kv : groups) {
+        int depth = kv.first;
+        const std::vector<int> &ids = kv.second;
+        std::vector<QuadBitBoard> roots(ids.size());
+        std::vector<u8> states(ids.size());
+        for (size_t k = 0; k < ids.size(); k++) {
+            memcpy(roots[k].bb, pos[ids[k]].bb, 32);
+            states[k] = pos[ids[k]].state;
+            if (pos[ids[k]].color == BLACK) flipColors(roots[k], states[k]);
+        }
+        int plies = forcedPlies == 1 || forcedPlies == 2 ? forcedPlies : chooseLeafPlies(depth, roots, states, useGraph);
+        if (plies >= depth) plies = depth - 1;
+        GroupPlan g{depth, plies, depth - plies};
+        std::vector<u64> out(ids.size());
+        PG_TRY(runGroupSplit(g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
+        for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];
+    }
+    return PG_OK;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+void pg_set_leaf_plies(int plies)
+{
+    if (plies == 1 || plies == 2) g_leafPlies = plies;
+}
+
+int pg_perft_batch(int n, const pg_position *positions, const uint8_t *depths, const pg_batch_options *opt,
+                   uint64_t *counts)
+{
+    if (n < 0 || (n && (!positions || !depths || !counts))) return PG_ERR_INVALID;
+    if (n == 0) return PG_OK;
+    return perftBatch(n, positions, depths, opt, counts);
+}
+
+}

@@ -1,0 +1,852 @@
+// pg_kernels.hip -- gfx950 kernels of the host-driven BFS and the plain
+// per-position driver (perft_gpu_host_bfs, perft_kernels.cu:620-990).
+//
+// One BFS level (entries = (parent index, move) pairs):
+//   k_makecount   child board = make(parent board, move); count its moves;
+//                 per-256-entry tile sums for the scan            (HBM bound)
+//   k_scan_top    exclusive scan of the tile sums (one block)
+//   k_scan_apply  wave64 block scan + tile offset -> inclusive counts
+//   k_partitions  one upper_bound per 2048-child tile
+//   k_expand      parent index of every child, window staged in LDS
+//   k_genmoves    moves of every board, staged in LDS, written coalesced
+// and after the last level a leaf kernel sums perft(1) or perft(2) of every
+// entry (k_leaf; the 2-ply form is the reference's fused leaf, here with the
+// child moves consumed as they are generated -- no per-thread move list).
+//
+// Indices are u32 (a level holds < 2^32 entries; larger levels report
+// PG_ERR_ARENA_OOM and the caller splits one ply down), sizes are 64-bit.
+#include <string.h>
+
+#include "pg_internal.h"
+#include "pg_device.h"
+#include "pg_tt.h"
+
+namespace pg {
+
+// ------------------------------------------------------------- level kernels
+template <int C>
+__global__ __launch_bounds__(BS) void k_makecount(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
+                                                  const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
+                                                  QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
+                                                  u32 *__restrict__ cnt, u32 *__restrict__ tileSums)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u32 c = 0;
+    if (i < n) {
+        u32 p = pidx[i];
+        QuadBitBoard q = pb[p];
+        u8 st = ps[p];
+        makeMove<C>(q, st, mv[i]);
+        ob[i] = q;
+        os[i] = st;
+        c = countMoves<1 - C>(q, st);
+        cnt[i] = c;
+    }
+    u64 s = blockSum64(c);
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)s;
+}
+
+// per-tile sums when the caller has none (pg_scan_u32 on arbitrary data)
+__global__ __launch_bounds__(BS) void k_tile_sums(const u32 *__restrict__ d, u64 n, u32 *__restrict__ tileSums)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u64 s = blockSum64(i < n ? d[i] : 0);
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)s;
+}
+
+// exclusive scan of `tiles` tile sums into tileOffs[0..tiles], tileOffs[tiles] = total
+constexpr int TOP_BS = 1024;
+__global__ __launch_bounds__(TOP_BS) void k_scan_top(const u32 *__restrict__ tileSums, u64 tiles,
+                                                     u64 *__restrict__ tileOffs)
+{
+    __shared__ u64 wtot[TOP_BS / 64];
+    __shared__ u64 carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (u64 base = 0; base < tiles; base += TOP_BS) {
+        u64 i = base + threadIdx.x;
+        u64 v = i < tiles ? tileSums[i] : 0;
+        u64 incl = waveInclScan64(v);
+        if (lane == 63) wtot[w] = incl;
+        __syncthreads();
+        u64 before = carry;
+        for (int k = 0; k < w; k++) before += wtot[k];
+        if (i < tiles) tileOffs[i] = before + incl - v;
+        __syncthreads();
+        if (threadIdx.x == TOP_BS - 1) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tileOffs[tiles] = carry;
+}
+
+__global__ __launch_bounds__(BS) void k_scan_apply(u32 *__restrict__ d, u64 n, const u64 *__restrict__ tileOffs)
+{
+    __shared__ u32 wtot[BS / 64];
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u32 v = i < n ? d[i] : 0;
+    u32 incl = waveInclScan(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    u32 before = 0;
+    for (int k = 0; k < w; k++) before += wtot[k];
+    if (i < n) d[i] = (u32)(tileOffs[blockIdx.x] + before + incl);
+}
+
+// parts[t] = parent of child t*EXPAND_TILE, for t in [0, tiles]
+__global__ __launch_bounds__(BS) void k_partitions(const u32 *__restrict__ incl, u64 n, u64 c, u64 tiles,
+                                                   u32 *__restrict__ parts)
+{
+    u64 t = (u64)blockIdx.x * BS + threadIdx.x;
+    if (t > tiles) return;
+    u64 child = t * EXPAND_TILE;
+    if (child > c) child = c;
+    u64 p = upperBound(incl, n, child);
+    parts[t] = (u32)(p < n ? p : n - 1);
+}
+
+__global__ __launch_bounds__(BS) void k_expand(const u32 *__restrict__ incl, u64 n, u64 c,
+                                               const u32 *__restrict__ parts, u32 *__restrict__ parentIdx)
+{
+    __shared__ u32 win[EXPAND_WCAP];
+    u64 c0 = (u64)blockIdx.x * EXPAND_TILE;
+    u64 p0 = parts[blockIdx.x], p1 = parts[blockIdx.x + 1];
+    u64 w = p1 - p0 + 1;
+    bool staged = w <= EXPAND_WCAP;
+    if (staged) {
+        for (u64 k = threadIdx.x; k < w; k += BS) win[k] = incl[p0 + k];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < EXPAND_TILE / BS; k++) {
+        u64 ch = c0 + (u64)k * BS + threadIdx.x;
+        if (ch >= c) break;
+        u64 lo = 0, hi = w;
+        if (staged) {
+            while (lo < hi) {
+                u64 mid = (lo + hi) >> 1;
+                if ((u64)win[mid] <= ch) lo = mid + 1;
+                else hi = mid;
+            }
+        } else {
+            while (lo < hi) {
+                u64 mid = (lo + hi) >> 1;
+                if ((u64)incl[p0 + mid] <= ch) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        parentIdx[ch] = (u32)(p0 + lo);
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(BS) void k_genmoves(const QuadBitBoard *__restrict__ boards, const u8 *__restrict__ states,
+                                                 const u32 *__restrict__ incl, u64 n, u16 *__restrict__ out)
+{
+    __shared__ u16 buf[GEN_LCAP];
+    u64 b0 = (u64)blockIdx.x * BS;
+    u64 i = b0 + threadIdx.x;
+    u64 bEnd = b0 + BS < n ? b0 + BS : n;
+    u64 blockStart = b0 ? incl[b0 - 1] : 0;
+    u64 total = (u64)incl[bEnd - 1] - blockStart;
+    u64 excl = 0;
+    if (i < n) excl = (i ? (u64)incl[i - 1] : 0);
+    if (total <= GEN_LCAP) {
+        if (i < n) {
+            u16 *dst = buf + (excl - blockStart);
+            generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) { dst[k] = m; });
+        }
+        __syncthreads();
+        u16 *o = out + blockStart;
+        for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
+    } else if (i < n) {
+        u16 *dst = out + excl;
+        generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) { dst[k] = m; });
+    }
+}
+
+// One-ply leaf (makeMove_and_perft_leaf_kernel, perft_kernels.cu:313-350):
+// perft(1) of make(parent, move) per entry.
+template <int C>
+__global__ __launch_bounds__(BS) void k_leaf1(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
+                                              const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
+                                              u64 *__restrict__ sum)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u32 v = 0;
+    if (i < n) {
+        u32 p = pidx[i];
+        QuadBitBoard q = pb[p];
+        u8 st = ps[p];
+        makeMove<C>(q, st, mv[i]);
+        v = countMoves<1 - C>(q, st);
+    }
+    u64 s = blockSum64(v);
+    if (threadIdx.x == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
+}
+
+// Fused two-ply leaf (fused_2level_leaf_kernel, perft_kernels.cu:382-480),
+// wave-cooperative: each wave takes L2_CHUNK entries, makes the child
+// positions (one lane each) and counts their moves, scans the counts across
+// the wave, stages every (child, grand-child move) pair in LDS, then all 64
+// lanes walk the flat pair list doing make + countMoves.  Every lane gets the
+// same number of pairs however uneven the children's move counts are, no
+// per-thread move list is kept (the reference's CMove[80] stack array and its
+// 80-move cap are gone), and generation and counting live in separate loops
+// so the counter's register budget is not shared with the generator's.
+// Pair lists longer than L2_CAP are processed in windows.
+constexpr int L2_CHUNK = 32;
+constexpr int L2_CAP = 1536;
+constexpr int L2_WAVES = 4;
+
+template <int C>
+__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__restrict__ pb,
+                                                         const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                         const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum)
+{
+    __shared__ QuadBitBoard sb[L2_WAVES][L2_CHUNK];
+    __shared__ u8 ss[L2_WAVES][L2_CHUNK];
+    __shared__ u32 sl[L2_WAVES][L2_CAP];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    u64 acc = 0;
+    const u64 stride = (u64)gridDim.x * L2_WAVES * L2_CHUNK;
+    for (u64 base = ((u64)blockIdx.x * L2_WAVES + w) * L2_CHUNK; base < n; base += stride) {
+        u32 cnt = 0;
+        u64 e = base + lane;
+        if (lane < L2_CHUNK && e < n) {
+            u32 p = pidx[e];
+            QuadBitBoard q = pb[p];
+            u8 st = ps[p];
+            makeMove<C>(q, st, mv[e]);
+            sb[w][lane] = q;
+            ss[w][lane] = st;
+            cnt = countMoves<1 - C>(q, st);
+        }
+        u32 incl = waveInclScan(cnt);
+        u32 off = incl - cnt;
+        u32 total = __shfl(incl, 63);
+        waveLdsSync();
+        for (u32 wb = 0; wb < total; wb += L2_CAP) {
+            if (cnt && off < wb + L2_CAP && off + cnt > wb) {
+                QuadBitBoard q = sb[w][lane];
+                u8 st = ss[w][lane];
+                u32 *dst = sl[w];
+                const u32 tag = (u32)lane << 16;
+                generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+                    u32 j = off + k - wb;  // wraps (and is skipped) below the window
+                    if (j < L2_CAP) dst[j] = (u32)m | tag;
+                });
+            }
+            waveLdsSync();
+            u32 wn = total - wb < (u32)L2_CAP ? total - wb : (u32)L2_CAP;
+            for (u32 j = lane; j < wn; j += 64) {
+                u32 x = sl[w][j];
+                u32 src = x >> 16;
+                QuadBitBoard c = sb[w][src];
+                u8 s2 = ss[w][src];
+                makeMove<1 - C>(c, s2, (u16)x);
+                acc += countMoves<C>(c, s2);
+            }
+            waveLdsSync();
+        }
+    }
+    u64 s = waveSum64(acc);
+    if (lane == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
+}
+
+// ------------------------------------------------------------ TT-mode kernels
+// With a transposition table each expanded level probes the table at its
+// remaining depth (a hit is a known subtree: no children are generated), the
+// leaf returns per-entry values, and an upsweep sums every parent's children
+// (a contiguous range, so no atomics) and stores the sums in the table
+// (reference: makemove_and_count_moves_kernel<useTT>, reduce_by_parent,
+// add_tt_hits_and_store, perft_kernels.cu:201-257, 554-599).
+
+// wave-aggregated probe/hit/store counters
+__device__ __forceinline__ void ttCount(u64 *ctr, bool probed, bool hit, bool stored)
+{
+    u64 pb = __ballot(probed), hb = __ballot(hit), sb = __ballot(stored);
+    if ((threadIdx.x & 63) == 0) {
+        if (pb) atomicAdd((unsigned long long *)&ctr[0], (unsigned long long)__popcll(pb));
+        if (hb) atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)__popcll(hb));
+        if (sb) atomicAdd((unsigned long long *)&ctr[2], (unsigned long long)__popcll(sb));
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(BS) void k_makecount_tt(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
+                                                     const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
+                                                     QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
+                                                     u32 *__restrict__ cnt, u32 *__restrict__ tileSums,
+                                                     u64 *__restrict__ known, const TTEntry *__restrict__ tt,
+                                                     u64 mask, int rd, u64 *ctr)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u32 c = 0;
+    bool hit = false;
+    if (i < n) {
+        u32 p = pidx[i];
+        QuadBitBoard q = pb[p];
+        u8 st = ps[p];
+        makeMove<C>(q, st, mv[i]);
+        ob[i] = q;
+        os[i] = st;
+        u64 v;
+        hit = ttProbe(tt, mask, hashPosition(q, st, 1 - C), rd, &v);
+        known[i] = hit ? v + 1 : 0;
+        c = hit ? 0 : countMoves<1 - C>(q, st);
+        cnt[i] = c;
+    }
+    ttCount(ctr, i < n, hit, false);
+    u64 s = blockSum64(c);
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)s;
+}
+
+// one-ply leaf with per-entry values
+template <int C>
+__global__ __launch_bounds__(BS) void k_leaf1_val(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
+                                                  const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
+                                                  u64 *__restrict__ val)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    u32 p = pidx[i];
+    QuadBitBoard q = pb[p];
+    u8 st = ps[p];
+    makeMove<C>(q, st, mv[i]);
+    val[i] = countMoves<1 - C>(q, st);
+}
+
+// two-ply leaf with per-entry values and the depth-2 table (the reference's
+// leaf TT, perft_kernels.cu:867-877): wave-cooperative like k_leaf2, with each
+// entry's partial sum kept in LDS
+template <int C>
+__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard *__restrict__ pb,
+                                                             const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                             const u16 *__restrict__ mv, u64 n, u64 *__restrict__ val,
+                                                             TTEntry *__restrict__ tt, u64 mask, u64 *ctr)
+{
+    __shared__ QuadBitBoard sb[L2_WAVES][L2_CHUNK];
+    __shared__ u8 ss[L2_WAVES][L2_CHUNK];
+    __shared__ u32 sacc[L2_WAVES][L2_CHUNK];
+    __shared__ u32 sl[L2_WAVES][L2_CAP];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const u64 stride = (u64)gridDim.x * L2_WAVES * L2_CHUNK;
+    for (u64 base = ((u64)blockIdx.x * L2_WAVES + w) * L2_CHUNK; base < n; base += stride) {
+        u32 cnt = 0;
+        u64 e = base + lane, hv = 0;
+        bool valid = lane < L2_CHUNK && e < n, hit = false;
+        if (valid) {
+            u32 p = pidx[e];
+            QuadBitBoard q = pb[p];
+            u8 st = ps[p];
+            makeMove<C>(q, st, mv[e]);
+            sb[w][lane] = q;
+            ss[w][lane] = st;
+            hit = tt && ttProbe(tt, mask, hashPosition(q, st, 1 - C), 2, &hv);
+            cnt = hit ? 0 : countMoves<1 - C>(q, st);
+        }
+        if (lane < L2_CHUNK) sacc[w][lane] = 0;
+        u32 incl = waveInclScan(cnt);
+        u32 off = incl - cnt;
+        u32 total = __shfl(incl, 63);
+        waveLdsSync();
+        for (u32 wb = 0; wb < total; wb += L2_CAP) {
+            if (cnt && off < wb + L2_CAP && off + cnt > wb) {
+                QuadBitBoard q = sb[w][lane];
+                u8 st = ss[w][lane];
+                u32 *dst = sl[w];
+                const u32 tag = (u32)lane << 16;
+                generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+                    u32 j = off + k - wb;
+                    if (j < L2_CAP) dst[j] = (u32)m | tag;
+                });
+            }
+            waveLdsSync();
+            u32 wn = total - wb < (u32)L2_CAP ? total - wb : (u32)L2_CAP;
+            for (u32 j = lane; j < wn; j += 64) {
+                u32 x = sl[w][j];
+                u32 src = x >> 16;
+                QuadBitBoard c = sb[w][src];
+                u8 s2 = ss[w][src];
+                makeMove<1 - C>(c, s2, (u16)x);
+                atomicAdd(&sacc[w][src], countMoves<C>(c, s2));
+            }
+            waveLdsSync();
+        }
+        bool store = false;
+        if (valid) {
+            u64 v = hit ? hv : sacc[w][lane];
+            val[e] = v;
+            if (tt && !hit) {
+                ttStore(tt, mask, hashPosition(sb[w][lane], ss[w][lane], 1 - C), 2, v);
+                store = true;
+            }
+        }
+        if (tt) ttCount(ctr, valid, hit, store);
+        waveLdsSync();
+    }
+}
+
+// upsweep of one level: value = table hit, or the sum of the children's
+// values; misses are stored at this level's remaining depth
+template <int C>  // side to move in this level's boards
+__global__ __launch_bounds__(BS) void k_upsweep(const QuadBitBoard *__restrict__ boards, const u8 *__restrict__ states,
+                                                const u32 *__restrict__ incl, const u64 *__restrict__ known,
+                                                const u64 *__restrict__ childVal, u64 n, u64 *__restrict__ val,
+                                                TTEntry *__restrict__ tt, u64 mask, int rd, u64 *ctr)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    bool store = false;
+    if (i < n) {
+        u64 v;
+        if (known[i]) {
+            v = known[i] - 1;
+        } else {
+            v = 0;
+            u64 b = i ? incl[i - 1] : 0, e = incl[i];
+            for (u64 k = b; k < e; k++) v += childVal[k];
+            if (tt) {
+                ttStore(tt, mask, hashPosition(boards[i], states[i], C), rd, v);
+                store = true;
+            }
+        }
+        val[i] = v;
+    }
+    if (tt) ttCount(ctr, false, false, store);
+}
+
+__global__ __launch_bounds__(BS) void k_sum_u64(const u64 *__restrict__ v, u64 n, u64 *__restrict__ sum)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u64 s = blockSum64(i < n ? v[i] : 0);
+    if (threadIdx.x == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
+}
+
+// ------------------------------------------------------------ launch helpers
+static inline unsigned gridFor(u64 n, int per) { return (unsigned)((n + per - 1) / per); }
+
+static int launchMakecount(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, const u16 *mv, int color, u64 n,
+                           QuadBitBoard *ob, u8 *os, u32 *cnt, u32 *tileSums, hipStream_t st)
+{
+    if (!n) return PG_OK;
+    if (color == WHITE)
+        hipLaunchKernelGGL(k_makecount<WHITE>, dim3(gridFor(n, BS)), dim3(BS), 0, st, pb, ps, pidx, mv, n, ob, os, cnt,
+                           tileSums);
+    else
+        hipLaunchKernelGGL(k_makecount<BLACK>, dim3(gridFor(n, BS)), dim3(BS), 0, st, pb, ps, pidx, mv, n, ob, os, cnt,
+                           tileSums);
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+// scan: tileSums (per 256) -> tileOffs -> in-place inclusive scan
+static int launchScan(u32 *d, u64 n, const u32 *tileSums, u64 *tileOffs, hipStream_t st)
+{
+    if (!n) return PG_OK;
+    u64 tiles = (n + BS - 1) / BS;
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_BS), 0, st, tileSums, tiles, tileOffs);
+    hipLaunchKernelGGL(k_scan_apply, dim3(gridFor(n, BS)), dim3(BS), 0, st, d, n, tileOffs);
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+static int launchExpand(const u32 *incl, u64 n, u64 c, u32 *parts, u32 *parentIdx, hipStream_t st)
+{
+    if (!c) return PG_OK;
+    u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
+    hipLaunchKernelGGL(k_partitions, dim3(gridFor(tiles + 1, BS)), dim3(BS), 0, st, incl, n, c, tiles, parts);
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)tiles), dim3(BS), 0, st, incl, n, c, parts, parentIdx);
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+static int launchGenmoves(const QuadBitBoard *boards, const u8 *states, const u32 *incl, int color, u64 n, u16 *out,
+                          hipStream_t st)
+{
+    if (!n) return PG_OK;
+    if (color == WHITE)
+        hipLaunchKernelGGL(k_genmoves<WHITE>, dim3(gridFor(n, BS)), dim3(BS), 0, st, boards, states, incl, n, out);
+    else
+        hipLaunchKernelGGL(k_genmoves<BLACK>, dim3(gridFor(n, BS)), dim3(BS), 0, st, boards, states, incl, n, out);
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+static int launchLeaf(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, const u16 *mv, int color, u64 n,
+                      int plies, u64 *sum, hipStream_t st)
+{
+    if (!n) return PG_OK;
+    if (plies == 1) {
+        dim3 g(gridFor(n, BS)), b(BS);
+        if (color == WHITE) hipLaunchKernelGGL(k_leaf1<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        else hipLaunchKernelGGL(k_leaf1<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+    } else {
+        // persistent-style grid: enough waves to fill every CU, each wave strides over chunks
+        u64 chunks = (n + L2_CHUNK - 1) / L2_CHUNK;
+        u64 blocks = (chunks + L2_WAVES - 1) / L2_WAVES;
+        u64 cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 8;
+        dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(L2_WAVES * 64);
+        if (color == WHITE) hipLaunchKernelGGL(k_leaf2<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        else hipLaunchKernelGGL(k_leaf2<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+    }
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+// --------------------------------------------------------- TT BFS driver
+// Same level loop as bfsPerft with table probes per level, per-entry leaf
+// values and an upsweep that stores every computed subtree.
+struct TTLevel {
+    const QuadBitBoard *boards;
+    const u8 *states;
+    u32 *incl;
+    u64 *known;
+    u64 *val;
+    u64 n;
+};
+
+static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, const DeviceTT &T,
+                      Bump &A, hipStream_t st, u64 *pinned, uint64_t *count)
+{
+    Lib &L = lib();
+    TTEntry *tt = (TTEntry *)T.entries;
+    u64 mask = T.mask;
+    u16 rootMoves[MAX_MOVES];
+    u32 n1 = generateMoves(root, state, color, rootMoves);
+    if (n1 == 0) {
+        *count = 0;
+        return PG_OK;
+    }
+    QuadBitBoard *b0 = A.alloc<QuadBitBoard>(1);
+    u8 *s0 = A.alloc<u8>(1);
+    u64 *dSum = A.alloc<u64>(1);
+    u64 *ctr = A.alloc<u64>(4);
+    u32 *idx = A.alloc<u32>(n1);
+    u16 *mv = A.alloc<u16>(n1);
+    if (!b0 || !s0 || !dSum || !ctr || !idx || !mv) return PG_ERR_ARENA_OOM;
+    PG_HIP(hipMemcpyAsync(b0, &root, sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(s0, &state, 1, hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(mv, rootMoves, n1 * sizeof(u16), hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemsetAsync(idx, 0, n1 * sizeof(u32), st));
+    PG_HIP(hipMemsetAsync(dSum, 0, sizeof(u64), st));
+    PG_HIP(hipMemsetAsync(ctr, 0, 4 * sizeof(u64), st));
+    std::vector<TTLevel> lv;
+    const QuadBitBoard *pb = b0;
+    const u8 *ps = s0;
+    u64 n = n1;
+    int last = depth - leafPlies;
+    for (int lvl = 1; lvl < last; lvl++) {
+        int pc = color ^ ((lvl - 1) & 1);
+        u64 tiles = (n + BS - 1) / BS;
+        QuadBitBoard *ob = A.alloc<QuadBitBoard>(n);
+        u8 *os = A.alloc<u8>(n);
+        u32 *cnt = A.alloc<u32>(n);
+        u64 *known = A.alloc<u64>(n);
+        u64 *val = A.alloc<u64>(n);
+        u32 *tileSums = A.alloc<u32>(tiles);
+        u64 *tileOffs = A.alloc<u64>(tiles + 1);
+        if (!ob || !os || !cnt || !known || !val || !tileSums || !tileOffs) return PG_ERR_ARENA_OOM;
+        dim3 g(gridFor(n, BS)), b(BS);
+        int rd = depth - lvl;
+        if (pc == WHITE)
+            hipLaunchKernelGGL(k_makecount_tt<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, cnt, tileSums, known, tt,
+                               mask, rd, ctr);
+        else
+            hipLaunchKernelGGL(k_makecount_tt<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, cnt, tileSums, known, tt,
+                               mask, rd, ctr);
+        PG_TRY(launchScan(cnt, n, tileSums, tileOffs, st));
+        PG_HIP(hipMemcpyAsync(pinned, tileOffs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        u64 c = pinned[0];
+        lv.push_back({ob, os, cnt, known, val, n});
+        if (c >= 0xFFFFFFFFull) return PG_ERR_ARENA_OOM;
+        u32 *parts = A.alloc<u32>((c + EXPAND_TILE - 1) / EXPAND_TILE + 1);
+        u32 *nidx = A.alloc<u32>(c);
+        u16 *nmv = A.alloc<u16>(c);
+        if (!parts || !nidx || !nmv) return PG_ERR_ARENA_OOM;
+        if (c) {
+            PG_TRY(launchExpand(cnt, n, c, parts, nidx, st));
+            PG_TRY(launchGenmoves(ob, os, cnt, pc ^ 1, n, nmv, st));
+        }
+        pb = ob;
+        ps = os;
+        idx = nidx;
+        mv = nmv;
+        n = c;
+    }
+    // leaf values
+    int pc = color ^ ((last - 1) & 1);
+    u64 *leafVal = A.alloc<u64>(n);
+    if (!leafVal) return PG_ERR_ARENA_OOM;
+    if (n) {
+        if (leafPlies == 1) {
+            dim3 g(gridFor(n, BS)), b(BS);
+            if (pc == WHITE) hipLaunchKernelGGL(k_leaf1_val<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, leafVal);
+            else hipLaunchKernelGGL(k_leaf1_val<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, leafVal);
+        } else {
+            u64 chunks = (n + L2_CHUNK - 1) / L2_CHUNK;
+            u64 blocks = (chunks + L2_WAVES - 1) / L2_WAVES;
+            u64 cap = (u64)(L.cuCount ? L.cuCount : 256) * 8;
+            dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(L2_WAVES * 64);
+            if (pc == WHITE) hipLaunchKernelGGL(k_leaf2_val<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, leafVal, tt, mask, ctr);
+            else hipLaunchKernelGGL(k_leaf2_val<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, leafVal, tt, mask, ctr);
+        }
+        PG_HIP(hipGetLastError());
+    }
+    // upsweep: level `last-1` down to 1
+    const u64 *childVal = leafVal;
+    for (int lvl = last - 1; lvl >= 1; lvl--) {
+        TTLevel &l = lv[lvl - 1];
+        int side = color ^ (lvl & 1);
+        dim3 g(gridFor(l.n, BS)), b(BS);
+        if (side == WHITE)
+            hipLaunchKernelGGL(k_upsweep<WHITE>, g, b, 0, st, l.boards, l.states, l.incl, l.known, childVal, l.n, l.val,
+                               tt, mask, depth - lvl, ctr);
+        else
+            hipLaunchKernelGGL(k_upsweep<BLACK>, g, b, 0, st, l.boards, l.states, l.incl, l.known, childVal, l.n, l.val,
+                               tt, mask, depth - lvl, ctr);
+        childVal = l.val;
+    }
+    hipLaunchKernelGGL(k_sum_u64, dim3(gridFor(n1, BS)), dim3(BS), 0, st, childVal, (u64)n1, dSum);
+    PG_HIP(hipGetLastError());
+    PG_HIP(hipMemcpyAsync(pinned, dSum, sizeof(u64), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipMemcpyAsync(pinned + 1, ctr, 3 * sizeof(u64), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    *count = pinned[0];
+    ttCountDevice(pinned[1], pinned[2], pinned[3]);
+    return PG_OK;
+}
+
+// --------------------------------------------------------------- BFS driver
+// perft_gpu_host_bfs (perft_kernels.cu:620-990): root moves on the host, then
+// levels 1 .. depth-leafPlies-1 on the device with one 8-byte readback of the
+// next level's size per level, then the leaf.  Returns PG_ERR_ARENA_OOM (and
+// no count) when a level does not fit; the caller recomputes one ply down.
+int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, bool useTT, void *arena,
+             size_t arenaBytes, hipStream_t st, u64 *pinned, uint64_t *count, size_t *peakBytes)
+{
+    if (depth <= 0) {
+        *count = 1;
+        return PG_OK;
+    }
+    if (depth == 1) {
+        *count = countMoves(root, state, color);
+        return PG_OK;
+    }
+    if (leafPlies < 1) leafPlies = 2;
+    if (leafPlies > depth - 1) leafPlies = depth - 1;
+    if (leafPlies > 2) leafPlies = 2;
+
+    u16 rootMoves[MAX_MOVES];
+    u32 n1 = generateMoves(root, state, color, rootMoves);
+    if (n1 == 0) {
+        *count = 0;
+        return PG_OK;
+    }
+    Bump A(arena, arenaBytes);
+    if (useTT && lib().tt.entries) {
+        int s = bfsPerftTT(root, state, color, depth, leafPlies, lib().tt, A, st, pinned, count);
+        if (peakBytes) *peakBytes = A.peak;
+        return s;
+    }
+    QuadBitBoard *b0 = A.alloc<QuadBitBoard>(1);
+    u8 *s0 = A.alloc<u8>(1);
+    u64 *dSum = A.alloc<u64>(1);
+    u32 *idx = A.alloc<u32>(n1);
+    u16 *mv = A.alloc<u16>(n1);
+    if (!b0 || !s0 || !dSum || !idx || !mv) return PG_ERR_ARENA_OOM;
+    Lib &L = lib();
+    PG_HIP(hipMemcpyAsync(b0, &root, sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(s0, &state, 1, hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(mv, rootMoves, n1 * sizeof(u16), hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemsetAsync(idx, 0, n1 * sizeof(u32), st));
+    PG_HIP(hipMemsetAsync(dSum, 0, sizeof(u64), st));
+
+    const QuadBitBoard *pb = b0;
+    const u8 *ps = s0;
+    u64 n = n1, nParents = 1;
+    Profiler &P = L.prof;
+    int last = depth - leafPlies;  // level whose entries feed the leaf
+    for (int lvl = 1; lvl < last; lvl++) {
+        int pc = color ^ ((lvl - 1) & 1);  // side to move in the parent boards
+        u64 tiles = (n + BS - 1) / BS;
+        QuadBitBoard *ob = A.alloc<QuadBitBoard>(n);
+        u8 *os = A.alloc<u8>(n);
+        u32 *cnt = A.alloc<u32>(n);
+        u32 *tileSums = A.alloc<u32>(tiles);
+        u64 *tileOffs = A.alloc<u64>(tiles + 1);
+        if (!ob || !os || !cnt || !tileSums || !tileOffs) return PG_ERR_ARENA_OOM;
+        // algorithmic bytes: 43N + 33P (make+count), 8N (scan)
+        P.begin(K_MAKECOUNT, st);
+        PG_TRY(launchMakecount(pb, ps, idx, mv, pc, n, ob, os, cnt, tileSums, st));
+        P.end(43.0 * n + 33.0 * nParents, (double)n, st);
+        P.begin(K_SCAN, st);
+        PG_TRY(launchScan(cnt, n, tileSums, tileOffs, st));
+        P.end(8.0 * n, (double)n, st);
+        PG_HIP(hipMemcpyAsync(pinned, tileOffs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        u64 c = pinned[0];
+        if (c == 0) {
+            *count = 0;
+            if (peakBytes) *peakBytes = A.peak;
+            P.collect();
+            return PG_OK;
+        }
+        if (c >= 0xFFFFFFFFull) return PG_ERR_ARENA_OOM;
+        u32 *parts = A.alloc<u32>((c + EXPAND_TILE - 1) / EXPAND_TILE + 1);
+        u32 *nidx = A.alloc<u32>(c);
+        u16 *nmv = A.alloc<u16>(c);
+        if (!parts || !nidx || !nmv) return PG_ERR_ARENA_OOM;
+        // 4N + 4C (expand), 37N + 2C (genmoves)
+        P.begin(K_EXPAND, st);
+        PG_TRY(launchExpand(cnt, n, c, parts, nidx, st));
+        P.end(4.0 * n + 4.0 * c, (double)c, st);
+        P.begin(K_GENMOVES, st);
+        PG_TRY(launchGenmoves(ob, os, cnt, pc ^ 1, n, nmv, st));
+        P.end(37.0 * n + 2.0 * c, (double)n, st);
+        pb = ob;
+        ps = os;
+        idx = nidx;
+        mv = nmv;
+        nParents = n;
+        n = c;
+    }
+    int pc = color ^ ((last - 1) & 1);
+    // leaf: 6K + 33P
+    P.begin(leafPlies == 1 ? K_LEAF1 : K_LEAF2, st);
+    PG_TRY(launchLeaf(pb, ps, idx, mv, pc, n, leafPlies, dSum, st));
+    P.end(6.0 * n + 33.0 * nParents, (double)n, st);
+    PG_HIP(hipMemcpyAsync(pinned, dSum, sizeof(u64), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    P.collect();
+    *count = pinned[0];
+    if (peakBytes) *peakBytes = A.peak;
+    return PG_OK;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+// ------------------------------------------------------- kernel-level C ABI
+static hipStream_t toStream(void *s) { return s ? (hipStream_t)s : lib().stream; }
+
+extern "C" int pg_level_makecount(const void *parentBoards, const uint8_t *parentStates, const uint32_t *parentIdx,
+                                  const uint16_t *moves, int color, uint64_t n, void *boards, uint8_t *states,
+                                  uint32_t *counts, uint32_t *tileSums, void *stream)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if ((color & ~1) || (n && (!parentBoards || !parentStates || !parentIdx || !moves || !boards || !states ||
+                               !counts || !tileSums))) {
+        setError("pg_level_makecount: bad argument");
+        return PG_ERR_INVALID;
+    }
+    return launchMakecount((const QuadBitBoard *)parentBoards, parentStates, parentIdx, moves, color, n,
+                           (QuadBitBoard *)boards, states, counts, tileSums, toStream(stream));
+}
+
+extern "C" int pg_scan_u32(uint32_t *data, uint64_t n, const uint32_t *tileSums, void *scratch, uint64_t *total,
+                           void *stream)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if (n && (!data || !scratch)) {
+        setError("pg_scan_u32: bad argument");
+        return PG_ERR_INVALID;
+    }
+    hipStream_t st = toStream(stream);
+    if (!n) {
+        if (total) *total = 0;
+        return PG_OK;
+    }
+    u64 tiles = (n + BS - 1) / BS;
+    u64 *offs = (u64 *)scratch;
+    const u32 *ts = tileSums;
+    if (!ts) {
+        // tile sums go after the offsets in scratch (caller sized it for both)
+        u32 *own = (u32 *)(offs + tiles + 1);
+        hipLaunchKernelGGL(k_tile_sums, dim3(gridFor(n, BS)), dim3(BS), 0, st, data, n, own);
+        ts = own;
+    }
+    PG_TRY(launchScan(data, n, ts, offs, st));
+    if (total) {
+        PG_HIP(hipMemcpyAsync(lib().hostPinned, offs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        *total = lib().hostPinned[0];
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_expand(const uint32_t *incl, uint64_t n, uint64_t c, uint32_t *parentIdx, void *scratch,
+                         void *stream)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if (c && (!incl || !n || !parentIdx || !scratch)) {
+        setError("pg_expand: bad argument");
+        return PG_ERR_INVALID;
+    }
+    return launchExpand(incl, n, c, (u32 *)scratch, parentIdx, toStream(stream));
+}
+
+extern "C" int pg_genmoves(const void *boards, const uint8_t *states, const uint32_t *incl, int color, uint64_t n,
+                           uint16_t *childMoves, void *stream)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if ((color & ~1) || (n && (!boards || !states || !incl || !childMoves))) {
+        setError("pg_genmoves: bad argument");
+        return PG_ERR_INVALID;
+    }
+    return launchGenmoves((const QuadBitBoard *)boards, states, incl, color, n, childMoves, toStream(stream));
+}
+
+extern "C" int pg_leaf(const void *parentBoards, const uint8_t *parentStates, const uint32_t *parentIdx,
+                       const uint16_t *moves, int color, uint64_t n, int plies, uint64_t *sum, void *stream)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if ((color & ~1) || (plies != 1 && plies != 2) || !sum ||
+        (n && (!parentBoards || !parentStates || !parentIdx || !moves))) {
+        setError("pg_leaf: bad argument");
+        return PG_ERR_INVALID;
+    }
+    return launchLeaf((const QuadBitBoard *)parentBoards, parentStates, parentIdx, moves, color, n, plies, sum,
+                      toStream(stream));
+}
+
+extern "C" int pg_profile_enable(int on)
+{
+    lib().prof.on = on != 0;
+    return PG_OK;
+}
+
+extern "C" int pg_profile_reset(void)
+{
+    lib().prof.reset();
+    return PG_OK;
+}
+
+extern "C" int pg_profile_get(pg_kernel_stat *out, int maxOut, int *nOut)
+{
+    if (!out || !nOut) return PG_ERR_INVALID;
+    Profiler &P = lib().prof;
+    int n = 0;
+    for (int k = 0; k < K_COUNT && n < maxOut; k++) {
+        if (!P.launches[k]) continue;
+        pg_kernel_stat &s = out[n++];
+        memset(&s, 0, sizeof s);
+        strncpy(s.name, KERNEL_NAMES[k], sizeof s.name - 1);
+        s.launches = P.launches[k];
+        s.ms = P.ms[k];
+        s.bytes = P.bytes[k];
+        s.items = P.items[k];
+    }
+    *nOut = n;
+    return PG_OK;
+}
+
+extern "C" int pg_device_synchronize(void)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    PG_HIP(hipDeviceSynchronize());
+    return PG_OK;
+}

@@ -1,0 +1,222 @@
+// pg_tt.hip -- transposition-table storage: the device table in HBM and the
+// host lossless tier (initTT / freeTT / clearDeviceTTs / printTTStats,
+// launcher.cu:128-531).
+#include <stdlib.h>
+#include <string.h>
+
+#include <atomic>
+#include <vector>
+
+#include "pg_launcher.h"
+#include "pg_tt.h"
+
+namespace pg {
+
+struct HostEntry {
+    u64 key;    // hash.hi
+    u64 lo;     // hash.lo (full verification)
+    u64 count;
+    u32 next;   // pool index, 0 = end of chain
+    u32 depth;
+};
+
+struct HostTT {
+    std::atomic<u32> *heads = nullptr;
+    u64 mask = 0;
+    HostEntry *pool = nullptr;
+    u64 poolSize = 0;
+    std::atomic<u64> used{1};
+    std::atomic<u64> probes{0}, hits{0}, stores{0};
+};
+static HostTT g_host;
+static std::atomic<u64> g_devProbes{0}, g_devHits{0}, g_devStores{0};
+
+static inline u64 hostBucket(const Hash128 &h, int depth) { return (h.lo ^ ((u64)depth * 0xC2B2AE3D27D4EB4FULL)) & g_host.mask; }
+
+bool hostTTProbe(const Hash128 &h, int depth, uint64_t *count)
+{
+    if (!g_host.heads) return false;
+    g_host.probes.fetch_add(1, std::memory_order_relaxed);
+    for (u32 i = g_host.heads[hostBucket(h, depth)].load(std::memory_order_acquire); i; i = g_host.pool[i].next) {
+        const HostEntry &e = g_host.pool[i];
+        if (e.key == h.hi && e.lo == h.lo && e.depth == (u32)depth) {
+            *count = e.count;
+            g_host.hits.fetch_add(1, std::memory_order_relaxed);
+            return true;
+        }
+    }
+    return false;
+}
+
+void hostTTStore(const Hash128 &h, int depth, uint64_t count)
+{
+    if (!g_host.heads) return;
+    u64 idx = g_host.used.fetch_add(1, std::memory_order_relaxed);
+    if (idx >= g_host.poolSize) return;  // pool full: drop (tt.h:211)
+    HostEntry &e = g_host.pool[idx];
+    e.key = h.hi;
+    e.lo = h.lo;
+    e.count = count;
+    e.depth = (u32)depth;
+    std::atomic<u32> &head = g_host.heads[hostBucket(h, depth)];
+    u32 old = head.load(std::memory_order_relaxed);
+    do {
+        e.next = old;
+    } while (!head.compare_exchange_weak(old, (u32)idx, std::memory_order_release, std::memory_order_relaxed));
+    g_host.stores.fetch_add(1, std::memory_order_relaxed);
+}
+
+static void hostFree()
+{
+    free(g_host.heads);
+    free(g_host.pool);
+    g_host.heads = nullptr;
+    g_host.pool = nullptr;
+    g_host.mask = 0;
+    g_host.poolSize = 0;
+    g_host.used = 1;
+}
+
+static int hostInit(uint64_t hostMB)
+{
+    hostFree();
+    if (hostMB == 0) return PG_OK;
+    u64 bytes = hostMB << 20;
+    // 1/8 of the budget for bucket heads, the rest for entries; pool index is u32
+    u64 buckets = 1;
+    while (buckets * 2 * sizeof(u32) <= bytes / 8) buckets *= 2;
+    u64 entries = (bytes - buckets * sizeof(u32)) / sizeof(HostEntry);
+    if (entries > 0xFFFFFFF0ull) entries = 0xFFFFFFF0ull;
+    g_host.heads = (std::atomic<u32> *)calloc(buckets, sizeof(u32));
+    g_host.pool = (HostEntry *)malloc(entries * sizeof(HostEntry));
+    if (!g_host.heads || !g_host.pool) {
+        hostFree();
+        setError("host TT allocation of %llu MB failed", (unsigned long long)hostMB);
+        fprintf(stderr, "warning: %s, host TT disabled\n", pg_last_error());
+        return PG_OK;  // launcher.cu:339-345: warn and run without the table
+    }
+    g_host.mask = buckets - 1;
+    g_host.poolSize = entries;
+    return PG_OK;
+}
+
+static int deviceInit(uint64_t deviceMB)
+{
+    Lib &L = lib();
+    if (L.tt.entries) {
+        (void)hipFree(L.tt.entries);
+        L.tt = DeviceTT{};
+    }
+    if (deviceMB == 0 || !L.init) return PG_OK;
+    u64 bytes = deviceMB << 20;
+    u64 slots = 1;
+    while (slots * 2 * 16 <= bytes) slots *= 2;
+    void *p = nullptr;
+    if (hipMalloc(&p, slots * 16) != hipSuccess) {
+        (void)hipGetLastError();
+        fprintf(stderr, "warning: device TT allocation of %llu MB failed, device TT disabled\n",
+                (unsigned long long)deviceMB);
+        return PG_OK;
+    }
+    PG_HIP(hipMemsetAsync(p, 0, slots * 16, L.stream));
+    PG_HIP(hipStreamSynchronize(L.stream));
+    L.tt.entries = p;
+    L.tt.mask = slots - 1;
+    return PG_OK;
+}
+
+int ttInit(uint64_t deviceMB, uint64_t hostMB)
+{
+    PG_TRY(hostInit(hostMB));
+    return deviceInit(deviceMB);
+}
+
+int ttFree()
+{
+    hostFree();
+    Lib &L = lib();
+    if (L.tt.entries) (void)hipFree(L.tt.entries);
+    L.tt = DeviceTT{};
+    return PG_OK;
+}
+
+int ttClear()
+{
+    Lib &L = lib();
+    if (L.tt.entries) {
+        PG_HIP(hipMemsetAsync(L.tt.entries, 0, (L.tt.mask + 1) * 16, L.stream));
+        PG_HIP(hipStreamSynchronize(L.stream));
+    }
+    if (g_host.heads) {
+        memset((void *)g_host.heads, 0, (g_host.mask + 1) * sizeof(u32));
+        g_host.used = 1;
+    }
+    return PG_OK;
+}
+
+void ttCountDevice(u64 probes, u64 hits, u64 stores)
+{
+    g_devProbes += probes;
+    g_devHits += hits;
+    g_devStores += stores;
+}
+
+int ttStats(uint64_t out[6])
+{
+    out[0] = g_devProbes;
+    out[1] = g_devHits;
+    out[2] = g_devStores;
+    out[3] = g_host.probes;
+    out[4] = g_host.hits;
+    out[5] = g_host.stores;
+    return PG_OK;
+}
+
+void u128::toDecimal(char *buf, int n) const
+{
+    if (hi == 0) {
+        snprintf(buf, n, "%llu", (unsigned long long)lo);
+        return;
+    }
+    u32 limbs[4] = {(u32)lo, (u32)(lo >> 32), (u32)hi, (u32)(hi >> 32)};
+    u32 groups[8];
+    int ng = 0;
+    for (;;) {
+        bool zero = true;
+        u64 rem = 0;
+        for (int i = 3; i >= 0; i--) {
+            u64 cur = (rem << 32) | limbs[i];
+            limbs[i] = (u32)(cur / 1000000000u);
+            rem = cur % 1000000000u;
+            if (limbs[i]) zero = false;
+        }
+        groups[ng++] = (u32)rem;
+        if (zero) break;
+    }
+    int off = snprintf(buf, n, "%u", groups[ng - 1]);
+    for (int i = ng - 2; i >= 0 && off < n; i--) off += snprintf(buf + off, n - off, "%09u", groups[i]);
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+int pg_tt_init(uint64_t deviceMB, uint64_t hostMB) { return ttInit(deviceMB, hostMB); }
+int pg_tt_clear(void) { return ttClear(); }
+int pg_tt_free(void) { return ttFree(); }
+int pg_tt_stats(uint64_t out[6])
+{
+    if (!out) return PG_ERR_INVALID;
+    return ttStats(out);
+}
+int pg_u128_to_string(uint64_t lo, uint64_t hi, char *buf, int n)
+{
+    if (!buf || n < 2) return PG_ERR_INVALID;
+    u128 v;
+    v.lo = lo;
+    v.hi = hi;
+    v.toDecimal(buf, n);
+    return PG_OK;
+}
+}

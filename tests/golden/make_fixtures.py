@@ -1,0 +1,93 @@
+"""Writes the committed workload fixtures and golden counts under tests/golden/.
+
+  python tests/golden/make_fixtures.py [--threads N]
+
+Positions come from the oracle's seeded random walk (SplitMix64 stream,
+canonically sorted legal moves, Lemire pick; oracle_random_position /
+oracle_random_request).  The engine's own generator must reproduce every FEN
+(tests/test_host.py checks that), so the product never needs this script.
+
+Golden counts come from the oracle (oracle/oracle.c), which is pinned against
+published perft values in tests/test_oracle.py.  Where a full-size count is
+beyond the oracle (config 4 at depth 8, the deep serving requests), the file
+keeps the FEN and depth and a reduced-depth count instead.
+
+Configs (sizes and seeds as the benchmark defines them):
+  c1  seed 1, 4 plies, perft 1..5
+  c2  seed 2, 8 plies (non-terminal), perft 1..7
+  c3  seeds 0..4095: plies U[6,40] then depth U[3,6] from the seed's stream
+  c4  seeds 100..163, 10 plies, depth 8 (golden at depth 5)
+  c5  request ids 0..9999: plies U[0,60] then depth U[3,7]; golden for every
+      request of depth <= 5 and the first 64 of depth 6
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+
+
+def dump(name, rows):
+    path = os.path.join(HERE, name)
+    with open(path, "w") as f:
+        for r in rows:
+            f.write(json.dumps(r, separators=(",", ":")) + "\n")
+    print(f"wrote {path} ({len(rows)} rows)", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
+    ap.add_argument("--only", default="")
+    a = ap.parse_args()
+    T = a.threads
+    only = set(a.only.split(",")) if a.only else {"c1", "c2", "c3", "c4", "c5"}
+    t0 = time.time()
+
+    if "c1" in only:
+        p = oracle.random_position(1, 4, True)
+        counts = [oracle.perft(p, d, T) for d in range(1, 6)]
+        dump("c1.jsonl", [{"seed": 1, "plies": 4, "fen": oracle.write_fen(p), "depth": 5, "counts": counts}])
+
+    if "c2" in only:
+        p = oracle.random_position(2, 8, True)
+        counts = []
+        for d in range(1, 8):
+            counts.append(oracle.perft(p, d, T))
+            print(f"c2 depth {d}: {counts[-1]} ({time.time() - t0:.0f}s)", flush=True)
+        dump("c2.jsonl", [{"seed": 2, "plies": 8, "fen": oracle.write_fen(p), "depth": 7, "counts": counts}])
+
+    if "c3" in only:
+        reqs = [oracle.random_request(s, (6, 40), (3, 6)) for s in range(4096)]
+        counts = oracle.perft_batch([r[0] for r in reqs], [r[2] for r in reqs], T)
+        dump("c3.jsonl", [{"seed": s, "plies": r[1], "fen": oracle.write_fen(r[0]), "depth": r[2], "count": c}
+                          for s, (r, c) in enumerate(zip(reqs, counts))])
+        print(f"c3 total nodes {sum(counts)} ({time.time() - t0:.0f}s)", flush=True)
+
+    if "c4" in only:
+        ps = [oracle.random_position(s, 10, True) for s in range(100, 164)]
+        counts = oracle.perft_batch(ps, [5] * len(ps), T)
+        dump("c4.jsonl", [{"seed": 100 + i, "plies": 10, "fen": oracle.write_fen(p), "depth": 8,
+                           "golden_depth": 5, "golden_count": c} for i, (p, c) in enumerate(zip(ps, counts))])
+
+    if "c5" in only:
+        reqs = [oracle.random_request(s, (0, 60), (3, 7)) for s in range(10000)]
+        pick = [i for i, r in enumerate(reqs) if r[2] <= 5]
+        pick += [i for i, r in enumerate(reqs) if r[2] == 6][:64]
+        counts = oracle.perft_batch([reqs[i][0] for i in pick], [reqs[i][2] for i in pick], T)
+        gold = dict(zip(pick, counts))
+        dump("c5.jsonl", [{"seed": s, "plies": r[1], "fen": oracle.write_fen(r[0]), "depth": r[2],
+                           **({"count": gold[s]} if s in gold else {})} for s, r in enumerate(reqs)])
+    print(f"done in {time.time() - t0:.0f}s", flush=True)
+
+
+if __name__ == "__main__":
+    main()

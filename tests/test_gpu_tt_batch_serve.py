@@ -1,0 +1,135 @@
+"""GPU tests of the transposition table, the batched path and the serving loop,
+each against the oracle's golden counts or the plain path (bit-exact)."""
+import time
+
+import pytest
+
+from conftest import load_json, load_jsonl
+
+pytestmark = pytest.mark.gpu
+
+KIWI = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+
+
+@pytest.fixture()
+def tt(gpu):
+    gpu.tt_init(1024, 256)
+    yield gpu
+    gpu.tt_free()
+
+
+@pytest.mark.parametrize("leaf", [1, 2])
+def test_tt_bfs_equals_no_tt(tt, leaf):
+    tt.set_leaf_plies(leaf)
+    try:
+        for p in load_json("known_positions.json")["positions"][:7]:
+            for d, c in enumerate(p["counts"], 1):
+                if c > 3_000_000_000 or d < 3:
+                    continue
+                pos = tt.parse_fen(p["fen"])
+                print(f"[gpu] tt leaf={leaf} {p['name']} d={d}", flush=True)
+                assert tt.perft_bfs(pos, d, use_tt=True) == c
+                assert tt.perft_bfs(pos, d, use_tt=True) == c  # second call hits the table
+        st = tt.tt_stats()
+        assert st[0] > 0 and st[1] > 0 and st[2] > 0, st
+    finally:
+        tt.set_leaf_plies(1)
+
+
+def test_tt_launcher_with_host_tier(tt):
+    r = load_jsonl("c2.jsonl")[0]
+    p = tt.parse_fen(r["fen"])
+    for ld in (3, 4):
+        assert tt.perft(p, 7, launch_depth=ld, use_tt=True) == r["counts"][6]
+    st = tt.tt_stats()
+    assert st[3] > 0 and st[4] > 0  # host tier probed and hit
+
+
+def test_tt_tiny_table_still_exact(gpu):
+    gpu.tt_init(1, 0)  # 64 Ki slots: constant overwrites and collisions of slots
+    try:
+        p = gpu.parse_fen(KIWI)
+        assert gpu.perft_bfs(p, 5, use_tt=True) == 193690690
+        assert gpu.perft_bfs(p, 5, use_tt=True) == 193690690
+    finally:
+        gpu.tt_free()
+
+
+@pytest.mark.parametrize("leaf", [1, 2])
+def test_batch_matches_golden_c3(gpu, leaf):
+    rows = load_jsonl("c3.jsonl")
+    rows = [r for r in rows if r["depth"] <= 5][:1500] + [r for r in rows if r["depth"] == 6][:40]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    got = gpu.perft_batch(ps, [r["depth"] for r in rows], leaf_plies=leaf)
+    assert got == [r["count"] for r in rows]
+
+
+def test_batch_graph_replay_and_eager_agree(gpu):
+    rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (3, 4)][:600]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    ds = [r["depth"] for r in rows]
+    want = [r["count"] for r in rows]
+    eager = gpu.perft_batch(ps, ds, leaf_plies=2, use_graph=False)
+    for rep in range(3):  # replays of the captured graphs, eager work in between
+        assert gpu.perft_batch(ps, ds, leaf_plies=2, use_graph=True) == want
+        assert gpu.perft_bfs(ps[rep], ds[rep]) == want[rep]
+    assert eager == want
+
+
+def test_batch_edge_cases(gpu):
+    start = gpu.parse_fen(gpu.START_FEN)
+    assert gpu.perft_batch([], []) == []
+    assert gpu.perft_batch([start], [0]) == [1]
+    assert gpu.perft_batch([start, start, start], [1, 2, 3]) == [20, 400, 8902]
+    black = gpu.parse_fen("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R b KQkq -")
+    # colour-flipped roots (black to move) in the same batch as white ones
+    white = gpu.parse_fen(KIWI)
+    assert gpu.perft_batch([white, black], [4, 4]) == [4085603, gpu.perft_bfs(black, 4)]
+
+
+def test_batch_oom_split(gpu):
+    # a batch whose levels exceed a small arena is split (halves, then the
+    # host split of a single root) until every piece fits; counts stay exact
+    rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] == 5][:64]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    gpu.lib().pg_set_arena_limit(64 << 20)
+    try:
+        got = gpu.perft_batch(ps, [5] * len(ps), leaf_plies=1)
+    finally:
+        gpu.lib().pg_set_arena_limit(0)
+    assert got == [r["count"] for r in rows]
+
+
+def _serve(gpu, reqs, budget=8 << 30, slots=4, interarrival=0.0):
+    import ctypes
+    cfg = gpu.ServeConfig(budget, 512 << 20, slots, 1)
+    h = ctypes.c_void_p()
+    assert gpu.lib().pg_serve_create(ctypes.byref(cfg), ctypes.byref(h)) == 0, gpu.last_error()
+    out = {}
+    try:
+        for rid, (p, d) in reqs.items():
+            assert gpu.lib().pg_serve_submit(h, rid, ctypes.byref(p), d) == 0
+            if interarrival:
+                time.sleep(interarrival)
+        res = (gpu.ServeResult * 64)()
+        n = ctypes.c_int()
+        deadline = time.time() + 300
+        while len(out) < len(reqs) and time.time() < deadline:
+            gpu.lib().pg_serve_poll(h, res, 64, 100, ctypes.byref(n))
+            for i in range(n.value):
+                assert res[i].status == 0
+                out[res[i].id] = res[i].count
+    finally:
+        gpu.lib().pg_serve_destroy(h)
+    return out
+
+
+def test_serving_alone_equals_in_mix(gpu):
+    rows = load_jsonl("c5.jsonl")
+    sample = [r for r in rows if "count" in r][:120]
+    reqs = {r["seed"]: (gpu.parse_fen(r["fen"]), r["depth"]) for r in sample}
+    want = {r["seed"]: r["count"] for r in sample}
+    mixed = _serve(gpu, reqs, slots=4)
+    assert mixed == want
+    for rid in list(reqs)[:10]:  # each alone, one slot, nothing else in flight
+        assert _serve(gpu, {rid: reqs[rid]}, slots=1) == {rid: want[rid]}

@@ -1,0 +1,141 @@
+"""CPU-side checks of the engine library: it loads and exports the whole C ABI,
+and its host code (FEN, generator, make, -cpu perft, workload walk) agrees
+with the oracle.  No GPU is touched here."""
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, load_json, load_jsonl
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "perft_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(pg_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(engine):
+    L = engine.lib()
+    names = _declared_symbols()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing
+
+
+def test_python_mirror_covers_the_abi(engine):
+    mirrored = {n for n, _, _ in engine._SIGS}
+    assert set(_declared_symbols()) <= mirrored
+
+
+def _to_engine(engine, orc_pos):
+    p = engine.Position()
+    for i in range(4):
+        p.bb[i] = orc_pos.bb[i]
+    p.state, p.color = orc_pos.state, orc_pos.color
+    return p
+
+
+def _canon(moves):
+    return sorted(m & ~(4 << 12) if (m >> 12) in (4, 12, 13, 14, 15) else m for m in moves)
+
+
+def _strip_flags(moves):
+    # the engine sets the double-push flag only when an EP capture can follow
+    return sorted((m & 0xFFF) | (((m >> 12) & 0xB) << 12 if (m >> 12) >= 8 else (((m >> 12) if (m >> 12) in (2, 3, 5) else 0) << 12))
+                  for m in moves)
+
+
+def test_fen_parse_matches_oracle(engine, orc):
+    for p in load_json("known_positions.json")["positions"]:
+        a, b = engine.parse_fen(p["fen"]), orc.parse_fen(p["fen"])
+        assert tuple(a.bb) == b.bb and a.state == b.state and a.color == b.color
+
+
+def test_invalid_fen_rejected(engine):
+    with pytest.raises(engine.PerftError):
+        engine.parse_fen("8/8/8/8/8/8/8/8 w - - 0 1")
+
+
+def test_generator_matches_oracle_on_random_positions(engine, orc):
+    for seed in range(4000):
+        o = orc.random_position(seed, seed % 100, require_nonterminal=False)
+        e = _to_engine(engine, o)
+        assert engine.count_moves(e) == orc.count_moves(o), orc.write_fen(o)
+        assert _strip_flags(engine.gen_moves(e)) == _strip_flags(orc.gen_moves(o)), orc.write_fen(o)
+
+
+def test_make_move_matches_oracle(engine, orc):
+    for seed in range(300):
+        o = orc.random_position(seed, seed % 60)
+        e = _to_engine(engine, o)
+        def key(m):
+            return (m & 0xFFF, ((m >> 12) & 3) if (m >> 12) >= 8 else -1)
+        by_key = {key(m): m for m in orc.gen_moves(o)}
+        for m in engine.gen_moves(e):
+            om = by_key[key(m)]
+            ec, oc = engine.make_move(e, m), orc.make_move(o, om)
+            assert tuple(ec.bb) == oc.bb and ec.color == oc.color
+            assert (ec.state & 15) == (oc.state & 15)
+            # same legal continuations (EP state may differ only when no EP capture exists)
+            assert engine.count_moves(ec) == orc.count_moves(oc)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3, 4])
+def test_cpu_perft_matches_known(engine, depth):
+    for p in load_json("known_positions.json")["positions"]:
+        if "counts" in p and len(p["counts"]) >= depth and p["counts"][depth - 1] < 5_000_000:
+            assert engine.perft_cpu(engine.parse_fen(p["fen"]), depth) == p["counts"][depth - 1]
+
+
+def test_cpu_perft_with_host_tt_matches(engine):
+    fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    engine.tt_init(0, 64)
+    try:
+        assert engine.perft_cpu(engine.parse_fen(fen), 4, use_tt=True) == 4085603
+        assert engine.perft_cpu(engine.parse_fen(fen), 4, use_tt=True) == 4085603
+        st = engine.tt_stats()
+        assert st[4] > 0 and st[5] > 0  # host hits and stores happened
+    finally:
+        engine.tt_free()
+
+
+def test_workload_walk_matches_oracle(engine, orc):
+    for s in range(200):
+        e, pl, d = engine.random_request(s, (6, 40), (3, 6))
+        o, pl2, d2 = orc.random_request(s, (6, 40), (3, 6))
+        assert (pl, d) == (pl2, d2)
+        assert engine.write_fen(e) == orc.write_fen(o)
+    for s in range(50):
+        assert engine.write_fen(engine.random_position(s, 30)) == orc.write_fen(orc.random_position(s, 30))
+
+
+@pytest.mark.parametrize("name", ["c1.jsonl", "c2.jsonl", "c3.jsonl", "c4.jsonl", "c5.jsonl"])
+def test_fixture_fens_reproduced_by_engine(engine, name):
+    rows = load_jsonl(name)
+    for r in rows[:400]:
+        if name in ("c1.jsonl", "c2.jsonl", "c4.jsonl"):
+            p = engine.random_position(r["seed"], r["plies"])
+        elif name == "c3.jsonl":
+            p, pl, d = engine.random_request(r["seed"], (6, 40), (3, 6))
+            assert (pl, d) == (r["plies"], r["depth"])
+        else:
+            p, pl, d = engine.random_request(r["seed"], (0, 60), (3, 7))
+            assert (pl, d) == (r["plies"], r["depth"])
+        assert engine.write_fen(p) == r["fen"]
+
+
+def test_cli_cpu_mode(engine):
+    r = engine.run_cli(["r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -", "3", "-cpu", "-nott"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "CPU mode" in r.stdout
+    assert engine.parse_cli_counts(r.stdout) == {1: 48, 2: 2039, 3: 97862}
+    assert re.search(r"Perft\(03\): 97862, time: [0-9.e+-]+ seconds", r.stdout)
+
+
+def test_u128_printing(engine):
+    import ctypes
+    buf = ctypes.create_string_buffer(64)
+    for v in [0, 1, 2**64 - 1, 2**64, 61885021521585529237, 2**128 - 1, 10**30]:
+        engine.lib().pg_u128_to_string(v & (2**64 - 1), v >> 64, buf, 64)
+        assert buf.value.decode() == str(v)
