@@ -18,7 +18,7 @@ import subprocess
 from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int, c_uint8, c_uint16, c_uint32, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libperft_gpu.so")
+LIB_PATH = os.environ.get("PERFT_GPU_LIB") or os.path.join(_HERE, "libperft_gpu.so")
 CLI_PATH = os.path.join(_HERE, "bin", "perft_gpu")
 
 PG_OK = 0

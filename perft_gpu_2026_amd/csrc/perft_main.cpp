@@ -4,12 +4,15 @@
 // with no arguments it runs the start position to depth 10.  Output lines keep
 // the reference's form: "Perft(%02d): <count>, time: <s> seconds, nps: <n>".
 // Extra flags of this engine: -leaf <1|2> (leaf plies of the GPU BFS),
-// -arena <MB> (BFS arena size).
+// -arena <MB> (BFS arena size), -divide (per-root-move counts at <depth>),
+// -suite <file> (EPD perft suite: "fen ;D1 n1 ;D2 n2 ..." per line).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
+#include <string>
+#include <vector>
 
 #include "../../include/perft_gpu.h"
 
@@ -32,14 +35,115 @@ static void printResult(int depth, uint64_t lo, uint64_t hi, double secs)
     fflush(stdout);
 }
 
+static void moveText(uint16_t m, char *buf)
+{
+    int f = m & 63, t = (m >> 6) & 63, fl = m >> 12;
+    buf[0] = (char)('a' + (f & 7));
+    buf[1] = (char)('1' + (f >> 3));
+    buf[2] = (char)('a' + (t & 7));
+    buf[3] = (char)('1' + (t >> 3));
+    buf[4] = (fl & 8) ? "nbrq"[fl & 3] : 0;
+    buf[5] = 0;
+}
+
+// one perft through the launcher (GPU) or the host recursion (-cpu)
+static int perftOne(const pg_position *p, int depth, int ld, bool cpu, bool tt, pg_u128 *out)
+{
+    if (cpu) {
+        uint64_t c = 0;
+        int s = pg_perft_cpu(p, depth, tt, &c);
+        out->lo = c;
+        out->hi = 0;
+        return s;
+    }
+    return pg_perft(p, depth, ld, tt, out);
+}
+
+static int runDivide(const pg_position *pos, int depth, int ld, bool cpu, bool tt)
+{
+    uint16_t mv[256];
+    int n = 0;
+    pg_gen_moves(pos, mv, &n);
+    unsigned __int128 total = 0;
+    double t0 = now();
+    for (int i = 0; i < n; i++) {
+        pg_position c = *pos;
+        pg_make_move(&c, mv[i]);
+        pg_u128 r{1, 0};
+        if (depth > 1 && perftOne(&c, depth - 1, ld < depth - 1 ? ld : depth - 1, cpu, tt, &r) != PG_OK) {
+            printf("perft failed: %s\n", pg_last_error());
+            return 1;
+        }
+        char mt[8], num[64];
+        moveText(mv[i], mt);
+        pg_u128_to_string(r.lo, r.hi, num, sizeof num);
+        printf("%s: %s\n", mt, num);
+        total += ((unsigned __int128)r.hi << 64) | r.lo;
+    }
+    char num[64];
+    pg_u128_to_string((uint64_t)total, (uint64_t)(total >> 64), num, sizeof num);
+    printf("\nMoves: %d\nTotal: %s, time: %g seconds\n", n, num, now() - t0);
+    return 0;
+}
+
+// EPD suite: "<fen> ;D1 20 ;D2 400 ..." per line; depths above maxDepth are skipped
+static int runSuite(const char *path, int maxDepth, bool cpu, bool tt)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        printf("cannot open %s\n", path);
+        return 1;
+    }
+    char line[4096];
+    int failures = 0, checks = 0;
+    while (fgets(line, sizeof line, f)) {
+        std::string l(line);
+        size_t semi = l.find(';');
+        if (semi == std::string::npos) continue;
+        std::string fen = l.substr(0, semi);
+        pg_position p;
+        if (pg_parse_fen(fen.c_str(), &p) != PG_OK) {
+            printf("bad FEN: %s\n", fen.c_str());
+            failures++;
+            continue;
+        }
+        size_t pos = semi;
+        while (pos != std::string::npos) {
+            int d = 0;
+            unsigned long long want = 0;
+            if (sscanf(l.c_str() + pos + 1, " D%d %llu", &d, &want) == 2 && d >= 1 && d <= maxDepth) {
+                pg_u128 r{0, 0};
+                int s = perftOne(&p, d, d, cpu, tt, &r);
+                bool ok = s == PG_OK && r.hi == 0 && r.lo == want;
+                printf("%s D%d %llu %s\n", ok ? "ok  " : "FAIL", d, (unsigned long long)r.lo, fen.c_str());
+                failures += !ok;
+                checks++;
+            }
+            pos = l.find(';', pos + 1);
+        }
+    }
+    fclose(f);
+    printf("\nsuite: %d checks, %d failures\n", checks, failures);
+    return failures ? 1 : 0;
+}
+
 int main(int argc, char *argv[])
 {
-    bool cpuMode = false, useTT = true;
+    bool cpuMode = false, useTT = true, divide = false;
+    const char *suite = nullptr;
     long ldOverride = 0, dttMB = 0, httMB = 0, arenaMB = 0, leaf = 0;
     // flags anywhere (perft.cu:17-57): consume them and compact argv
     for (int i = 1; i < argc; i++) {
         long *valueFlag = nullptr;
         if (!strcmp(argv[i], "-cpu")) cpuMode = true;
+        else if (!strcmp(argv[i], "-divide")) divide = true;
+        else if (!strcmp(argv[i], "-suite") && i + 1 < argc) {
+            suite = argv[i + 1];
+            for (int j = i; j + 2 < argc; j++) argv[j] = argv[j + 2];
+            argc -= 2;
+            i--;
+            continue;
+        }
         else if (!strcmp(argv[i], "-nott")) useTT = false;
         else if (!strcmp(argv[i], "-ld")) valueFlag = &ldOverride;
         else if (!strcmp(argv[i], "-dtt")) valueFlag = &dttMB;
@@ -74,6 +178,15 @@ int main(int argc, char *argv[])
     }
     if (leaf == 1 || leaf == 2) pg_set_leaf_plies((int)leaf);
 
+    if (suite) {
+        int maxD = argc >= 2 ? atoi(argv[1]) : 6;
+        if (useTT) pg_tt_init(cpuMode ? 0 : 4096, 1024);
+        int rc = runSuite(suite, maxD > 0 ? maxD : 6, cpuMode, useTT);
+        pg_tt_free();
+        if (!cpuMode) pg_shutdown();
+        return rc;
+    }
+
     const char *fen = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1";
     int maxDepth = 10;
     if (argc >= 3) {
@@ -91,6 +204,14 @@ int main(int argc, char *argv[])
     if (maxDepth < 0 || maxDepth > 20) {
         printf("depth %d out of range\n", maxDepth);
         return 1;
+    }
+
+    if (divide) {
+        if (useTT) pg_tt_init(cpuMode ? 0 : 4096, 1024);
+        int rc = runDivide(&pos, maxDepth, ldOverride > 0 ? (int)ldOverride : maxDepth, cpuMode, useTT);
+        pg_tt_free();
+        if (!cpuMode) pg_shutdown();
+        return rc;
     }
 
     if (cpuMode) {

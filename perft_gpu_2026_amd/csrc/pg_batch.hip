@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "pg_device.h"
@@ -48,6 +49,8 @@ struct LvlDev {
     u32 *tileSums;
     u64 *tileOffs;
     u32 *parts;            // expand partitions of the next level
+    u64 *known;            // TT mode: table hit + 1 (0 = miss)
+    u64 *val;              // TT mode: subtree count of each entry
 };
 
 struct BatchDev {
@@ -56,6 +59,10 @@ struct BatchDev {
     u32 oom, leafLevel;
     u64 work;                // leaf work-queue cursor
     u64 *rootCounts;
+    TTEntry *tt;             // shared device table (TT mode)
+    u64 ttMask;
+    u32 depth, useTT;
+    u64 ttCtr[3];            // probes, hits, stores
     LvlDev lv[MAXLV];
 };
 
@@ -79,7 +86,21 @@ __global__ void k_b_init(BatchDev *d)
     const u64 nRoots = d->lv[0].n;
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < nRoots; i += (u64)gridDim.x * blockDim.x) d->rootCounts[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) d->work = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d->work = 0;
+        d->ttCtr[0] = d->ttCtr[1] = d->ttCtr[2] = 0;
+    }
+}
+
+// wave-aggregated TT counters
+__device__ __forceinline__ void bttCount(BatchDev *d, bool probed, bool hit, bool stored)
+{
+    u64 pb = __ballot(probed), hb = __ballot(hit), sb = __ballot(stored);
+    if ((threadIdx.x & 63) == 0) {
+        if (pb) atomicAdd((unsigned long long *)&d->ttCtr[0], (unsigned long long)__popcll(pb));
+        if (hb) atomicAdd((unsigned long long *)&d->ttCtr[1], (unsigned long long)__popcll(hb));
+        if (sb) atomicAdd((unsigned long long *)&d->ttCtr[2], (unsigned long long)__popcll(sb));
+    }
 }
 
 __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
@@ -90,10 +111,17 @@ __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u64 i = t * BS + threadIdx.x;
         u32 c = 0;
+        bool hit = false;
         if (i < a.n) {
-            c = countMoves<WHITE>(a.boards[i], a.states[i]);
+            if (d->useTT) {
+                u64 v;
+                hit = ttProbe(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], WHITE), (int)d->depth, &v);
+                a.known[i] = hit ? v + 1 : 0;
+            }
+            c = hit ? 0 : countMoves<WHITE>(a.boards[i], a.states[i]);
             a.cnt[i] = c;
         }
+        if (d->useTT) bttCount(d, i < a.n, hit, false);
         u64 s = blockSum64(c);
         if (threadIdx.x == 0) a.tileSums[t] = (u32)s;
         __syncthreads();
@@ -117,7 +145,13 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
             makeMove<C>(q, st, a.mv[i]);
             a.boards[i] = q;
             a.states[i] = st;
-            c = countMoves<1 - C>(q, st);
+            bool hit = false;
+            if (d->useTT) {
+                u64 v;
+                hit = ttProbe(d->tt, d->ttMask, hashPosition(q, st, 1 - C), (int)d->depth - L, &v);
+                a.known[i] = hit ? v + 1 : 0;
+            }
+            c = hit ? 0 : countMoves<1 - C>(q, st);
             a.cnt[i] = c;
         }
         u64 s = blockSum64(c);
@@ -196,7 +230,9 @@ __global__ void k_b_alloc(BatchDev *d, int L)
         b.cnt = (u32 *)dalloc(d, 4 * c + 4);
         b.tileSums = (u32 *)dalloc(d, 4 * bt + 4);
         b.tileOffs = (u64 *)dalloc(d, 8 * (bt + 1));
+        if (d->useTT) b.known = (u64 *)dalloc(d, 8 * c + 8);
     }
+    if (d->useTT) b.val = (u64 *)dalloc(d, 8 * c + 8);
     if (!d->oom) b.n = c;
 }
 
@@ -396,16 +432,150 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
     }
 }
 
+// ---------------------------------------------------------- TT-mode kernels
+// Leaf values per entry (instead of per-root atomics), then an upsweep that
+// sums each parent's contiguous child range, stores the subtree in the table,
+// and finally the roots' sums.  All sizes come from the descriptor.
+template <int C>
+__global__ __launch_bounds__(BS) void k_b_leaf1v(BatchDev *d)
+{
+    if (d->oom) return;
+    const u32 L = d->leafLevel;
+    const LvlDev &a = d->lv[L];
+    const LvlDev &p = d->lv[L - 1];
+    for (u64 e = (u64)blockIdx.x * BS + threadIdx.x; e < a.n; e += (u64)gridDim.x * BS) {
+        u32 pi = a.idx[e];
+        QuadBitBoard q = p.boards[pi];
+        u8 st = p.states[pi];
+        makeMove<C>(q, st, a.mv[e]);
+        a.val[e] = countMoves<1 - C>(q, st);
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
+{
+    if (d->oom) return;
+    __shared__ QuadBitBoard sb[BL2_WAVES][B_CHUNK];
+    __shared__ u8 ss[BL2_WAVES][B_CHUNK];
+    __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
+    __shared__ u32 sl[BL2_WAVES][BL2_CAP];
+    const u32 L = d->leafLevel;
+    const LvlDev &a = d->lv[L];
+    const LvlDev &p = d->lv[L - 1];
+    const u64 n = a.n;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (;;) {
+        u64 base = 0;
+        if (lane == 0) base = atomicAdd((unsigned long long *)&d->work, (unsigned long long)B_CHUNK);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        u32 cnt = 0;
+        u64 e = base + lane, hv = 0;
+        bool valid = lane < B_CHUNK && e < n, hit = false;
+        if (valid) {
+            u32 pi = a.idx[e];
+            QuadBitBoard q = p.boards[pi];
+            u8 st = p.states[pi];
+            makeMove<C>(q, st, a.mv[e]);
+            sb[w][lane] = q;
+            ss[w][lane] = st;
+            hit = ttProbe(d->tt, d->ttMask, hashPosition(q, st, 1 - C), 2, &hv);
+            cnt = hit ? 0 : countMoves<1 - C>(q, st);
+        }
+        if (lane < B_CHUNK) sacc[w][lane] = 0;
+        u32 incl = waveInclScan(cnt);
+        u32 off = incl - cnt;
+        u32 total = __shfl(incl, 63);
+        waveLdsSync();
+        for (u32 wb = 0; wb < total; wb += BL2_CAP) {
+            if (cnt && off < wb + BL2_CAP && off + cnt > wb) {
+                QuadBitBoard q = sb[w][lane];
+                u8 st = ss[w][lane];
+                u32 *dst = sl[w];
+                const u32 tag = (u32)lane << 16;
+                generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+                    u32 j = off + k - wb;
+                    if (j < BL2_CAP) dst[j] = (u32)m | tag;
+                });
+            }
+            waveLdsSync();
+            u32 wn = total - wb < (u32)BL2_CAP ? total - wb : (u32)BL2_CAP;
+            for (u32 j = lane; j < wn; j += 64) {
+                u32 x = sl[w][j];
+                u32 src = x >> 16;
+                QuadBitBoard c = sb[w][src];
+                u8 s2 = ss[w][src];
+                makeMove<1 - C>(c, s2, (u16)x);
+                atomicAdd(&sacc[w][src], countMoves<C>(c, s2));
+            }
+            waveLdsSync();
+        }
+        bool store = valid && !hit;
+        if (valid) {
+            u64 v = hit ? hv : sacc[w][lane];
+            a.val[e] = v;
+            if (store) ttStore(d->tt, d->ttMask, hashPosition(sb[w][lane], ss[w][lane], 1 - C), 2, v);
+        }
+        bttCount(d, valid, hit, store);
+        waveLdsSync();
+    }
+}
+
+template <int C>  // side to move in level-L boards
+__global__ __launch_bounds__(BS) void k_b_upsweep(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    LvlDev &a = d->lv[L];
+    const u64 *cv = d->lv[L + 1].val;
+    const int rd = (int)d->depth - L;
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
+        u64 v;
+        bool store = false;
+        if (a.known[i]) {
+            v = a.known[i] - 1;
+        } else {
+            v = 0;
+            u64 b = i ? a.cnt[i - 1] : 0, e = a.cnt[i];
+            for (u64 k = b; k < e; k++) v += cv[k];
+            ttStore(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], C), rd, v);
+            store = true;
+        }
+        a.val[i] = v;
+        bttCount(d, false, false, store);
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
+{
+    if (d->oom) return;
+    const LvlDev &a = d->lv[0];
+    const u64 *cv = d->lv[1].val;
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
+        u64 v;
+        if (a.known[i]) {
+            v = a.known[i] - 1;
+        } else {
+            v = 0;
+            u64 b = i ? a.cnt[i - 1] : 0, e = a.cnt[i];
+            for (u64 k = b; k < e; k++) v += cv[k];
+            ttStore(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], WHITE), (int)d->depth, v);
+        }
+        d->rootCounts[i] = v;
+    }
+}
+
 // ------------------------------------------------------------------ host
 struct GroupPlan {
     int depth, plies, leafLevel;
+    bool useTT;
 };
 
 struct BatchState {
     BatchDev *dev = nullptr;  // device descriptor
     BatchDev host{};          // host copy used to initialise it
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (depth, plies) -> graph
-    std::map<std::pair<int, int>, int> tuned;                // (depth, size bucket) -> plies
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (depth, plies, tt) -> graph
+    std::map<std::tuple<int, int, int>, int> tuned;          // (depth, size bucket, tt) -> plies
 };
 static BatchState g_batch;
 
@@ -438,7 +608,20 @@ static int enqueueGroup(const GroupPlan &g, hipStream_t st)
         else hipLaunchKernelGGL(k_b_genmoves<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
     }
     int pside = (g.leafLevel - 1) & 1;  // side to move in the leaf entries' parents
-    if (g.plies == 1) {
+    if (g.useTT) {
+        if (g.plies == 1) {
+            if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1v<WHITE>, dim3(G), dim3(BS), 0, st, d);
+            else hipLaunchKernelGGL(k_b_leaf1v<BLACK>, dim3(G), dim3(BS), 0, st, d);
+        } else {
+            if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2v<WHITE>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+            else hipLaunchKernelGGL(k_b_leaf2v<BLACK>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+        }
+        for (int L = g.leafLevel - 1; L >= 1; L--) {
+            if ((L & 1) == WHITE) hipLaunchKernelGGL(k_b_upsweep<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
+            else hipLaunchKernelGGL(k_b_upsweep<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+        }
+        hipLaunchKernelGGL(k_b_rootsum, dim3(G), dim3(BS), 0, st, d);
+    } else if (g.plies == 1) {
         if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1<WHITE>, dim3(G), dim3(BS), 0, st, d);
         else hipLaunchKernelGGL(k_b_leaf1<BLACK>, dim3(G), dim3(BS), 0, st, d);
     } else {
@@ -475,7 +658,8 @@ static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *sta
     u32 *ts0 = A.alloc<u32>(tiles0);
     u64 *to0 = A.alloc<u64>(tiles0 + 1);
     u64 *rc = A.alloc<u64>(nRoots);
-    if (!dRoots || !dStates || !cnt0 || !ts0 || !to0 || !rc) return PG_ERR_ARENA_OOM;
+    u64 *known0 = g.useTT ? A.alloc<u64>(nRoots) : nullptr;
+    if (!dRoots || !dStates || !cnt0 || !ts0 || !to0 || !rc || (g.useTT && !known0)) return PG_ERR_ARENA_OOM;
     PG_HIP(hipMemcpyAsync(dRoots, roots, nRoots * sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
     PG_HIP(hipMemcpyAsync(dStates, states, nRoots, hipMemcpyHostToDevice, st));
     BatchDev &h = g_batch.host;
@@ -490,11 +674,16 @@ static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *sta
     h.lv[0].cnt = cnt0;
     h.lv[0].tileSums = ts0;
     h.lv[0].tileOffs = to0;
+    h.lv[0].known = known0;
+    h.depth = (u32)g.depth;
+    h.useTT = g.useTT ? 1 : 0;
+    h.tt = (TTEntry *)L.tt.entries;
+    h.ttMask = L.tt.mask;
     PG_HIP(hipMemcpyAsync(g_batch.dev, &h, sizeof h, hipMemcpyHostToDevice, st));
     Profiler &P = L.prof;
     P.begin(K_BATCH_GROUP, st);
     if (useGraph) {
-        auto key = std::make_pair(g.depth, g.plies);
+        auto key = std::make_tuple(g.depth, g.plies, (int)g.useTT);
         auto it = g_batch.graphs.find(key);
         if (it == g_batch.graphs.end()) {
             hipGraph_t graph;
@@ -515,8 +704,10 @@ static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *sta
     // one readback for the whole group: the OOM flag, then the counts
     PG_HIP(hipMemcpyAsync(L.hostPinned, &g_batch.dev->oom, sizeof(u32), hipMemcpyDeviceToHost, st));
     PG_HIP(hipMemcpyAsync(counts, rc, nRoots * sizeof(u64), hipMemcpyDeviceToHost, st));
+    if (g.useTT) PG_HIP(hipMemcpyAsync(L.hostPinned + 1, g_batch.dev->ttCtr, 3 * sizeof(u64), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
     P.collect();
+    if (g.useTT) ttCountDevice(L.hostPinned[1], L.hostPinned[2], L.hostPinned[3]);
     if (*(u32 *)L.hostPinned) return PG_ERR_ARENA_OOM;
     return PG_OK;
 }
@@ -541,7 +732,7 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
             flipColors(kids[i], ks[i]);
         }
         int plies = g.plies < g.depth - 1 ? g.plies : g.depth - 2;
-        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies};
+        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies, g.useTT};
         std::vector<u64> kc(nc);
         if (nc) PG_TRY(runGroupSplit(sub, kids.data(), ks.data(), nc, useGraph, kc.data()));
         u64 total = 0;
@@ -555,11 +746,11 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
 }
 
 static int chooseLeafPlies(int depth, const std::vector<QuadBitBoard> &roots, const std::vector<u8> &states,
-                           bool useGraph)
+                           bool useGraph, bool useTT)
 {
     int bucket = 0;
     for (u64 n = roots.size(); n > 1; n >>= 1) bucket++;
-    auto key = std::make_pair(depth, bucket);
+    auto key = std::make_tuple(depth, bucket, (int)useTT);
     auto it = g_batch.tuned.find(key);
     if (it != g_batch.tuned.end()) return it->second;
     // time both leaf forms on a sample of the group (second run of each, so
@@ -569,7 +760,7 @@ static int chooseLeafPlies(int depth, const std::vector<QuadBitBoard> &roots, co
     double best = 1e30;
     int bestP = 2;
     for (int p = 1; p <= 2 && p < depth; p++) {
-        GroupPlan g{depth, p, depth - p};
+        GroupPlan g{depth, p, depth - p, useTT};
         double t = 0;
         for (int rep = 0; rep < 2; rep++) {
             auto t0 = std::chrono::steady_clock::now();
@@ -595,6 +786,7 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
     std::lock_guard<std::mutex> lock(L.mu);
     if (!g_batch.dev) PG_HIP(hipMalloc((void **)&g_batch.dev, sizeof(BatchDev)));
     bool useGraph = !opt || opt->useGraph;
+    bool useTT = opt && opt->useTT && L.tt.entries;
     int forcedPlies = opt ? opt->leafPlies : 0;
     // group by depth; depth <= 1 on the host as the reference does (perft_kernels.cu:623-626)
     std::map<int, std::vector<int>> groups;
@@ -622,9 +814,10 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
             states[k] = pos[ids[k]].state;
             if (pos[ids[k]].color == BLACK) flipColors(roots[k], states[k]);
         }
-        int plies = forcedPlies == 1 || forcedPlies == 2 ? forcedPlies : chooseLeafPlies(depth, roots, states, useGraph);
+        int plies = forcedPlies == 1 || forcedPlies == 2 ? forcedPlies
+                                                          : chooseLeafPlies(depth, roots, states, useGraph, useTT);
         if (plies >= depth) plies = depth - 1;
-        GroupPlan g{depth, plies, depth - plies};
+        GroupPlan g{depth, plies, depth - plies, useTT};
         std::vector<u64> out(ids.size());
         PG_TRY(runGroupSplit(g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
         for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];

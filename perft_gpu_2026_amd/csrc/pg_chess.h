@@ -16,9 +16,9 @@
 //     occluded fills for whole piece sets, and hyperbola quintessence with a
 //     full 64-bit bit reverse (v_bfrev_b32 x2) for single-square lines;
 //     no table reads, so the hot loop is pure VALU.
-//   * pins and checks come from the four lines through the king, split by
-//     direction; pinned pieces are handled by per-line source masks instead of
-//     per-piece loops (a piece pinned on a line may move only along it).
+//   * pins and checks come from the few enemy sliders on the king's lines;
+//     pinned pieces are handled by per-line source masks instead of per-piece
+//     loops (a piece pinned on a line may move only along it).
 //   * legal-move counting is set-wise: knight, slider and pawn counts are
 //     popcounts of per-direction target sets; only en passant (rare) branches.
 // The semantics follow the reference's countMoves/makeMove
@@ -203,7 +203,7 @@ struct Frame {
     Sets s;
     int k;           // own king square
     u64 kBit;
-    u64 att;         // squares attacked by the enemy, own king transparent
+    u64 att;         // squares attacked by the enemy (x-ray through the king on check lines)
     u64 checkers;
     u64 checkMask;   // landing squares that resolve the check (all if none, none if double)
     u64 pinFile, pinRank, pinDiag, pinAnti;  // pin rays (king..pinner] per line
@@ -211,44 +211,85 @@ struct Frame {
     int nCheck;
 };
 
-// One line through the king: pins and slider checks in both directions.
-PG_HD void kingLine(u64 occ, u64 us, u64 sliders, u64 m, u64 kb, u64 up, u64 &pin, u64 &chk)
-{
-    u64 a = lineAttacks(occ, m, kb);
-    u64 x = lineAttacks(occ & ~(a & us), m, kb);  // x-ray through our first blockers
-    u64 xu = x & up, xd = x & ~up;
-    u64 au = a & up, ad = a & ~up;
-    pin |= sel((xu & sliders & ~au) != 0, xu, 0) | sel((xd & sliders & ~ad) != 0, xd, 0);
-    chk |= sel((au & sliders) != 0, au, 0) | sel((ad & sliders) != 0, ad, 0);
-}
-
+// Pins and checks from the "snipers": enemy sliders on the king's four
+// empty-board lines.  For each sniper, the squares strictly between it and the
+// king decide: none occupied -> it gives check (check ray = between + sniper,
+// and the square behind the king on that line is attacked through the king);
+// exactly one occupied and ours -> that piece is pinned to the line.  Snipers
+// are few (usually 0-3), so this beats scanning all four lines both ways.
+// `att` is left to the caller (countMoves fuses it with its own fills).
 template <int C>
-PG_HD Frame frame(const QuadBitBoard &q)
+PG_HD Frame framePins(const QuadBitBoard &q)
 {
     Frame f;
     f.s = derive<C>(q);
     const Sets &s = f.s;
     f.kBit = s.kings & s.us;
     f.k = lsb(f.kBit);
+    const int k = f.k;
     u64 eDiag = s.diag & s.them, eOrth = s.orth & s.them;
-    u64 emptyX = ~s.occ | f.kBit;
-    f.att = pawnAttacks<1 - C>(s.pawns & s.them) | knightAttacks(s.knights & s.them) |
-            kingAttacks(s.kings & s.them) | orthAttacksSet(eOrth, emptyX) | diagAttacksSet(eDiag, emptyX);
-
-    u64 up = (~1ULL) << f.k;  // squares above the king
-    u64 chkRays = 0;
+    u64 mFile = fileMaskEx(k), mRank = rankMaskEx(k), mDiag = diagMaskEx(k), mAnti = antiMaskEx(k);
+    u64 snipers = ((mFile | mRank) & eOrth) | ((mDiag | mAnti) & eDiag);
+    u64 below = f.kBit - 1, above = ~below & ~f.kBit;
+    u64 kAtt = kingAttacks(f.kBit);
+    u64 chkRays = 0, behind = 0;
     f.pinFile = f.pinRank = f.pinDiag = f.pinAnti = 0;
-    kingLine(s.occ, s.us, eOrth, fileMaskEx(f.k), f.kBit, up, f.pinFile, chkRays);
-    kingLine(s.occ, s.us, eOrth, rankMaskEx(f.k), f.kBit, up, f.pinRank, chkRays);
-    kingLine(s.occ, s.us, eDiag, diagMaskEx(f.k), f.kBit, up, f.pinDiag, chkRays);
-    kingLine(s.occ, s.us, eDiag, antiMaskEx(f.k), f.kBit, up, f.pinAnti, chkRays);
-
+    while (snipers) {
+        u64 sb = snipers & (0 - snipers);
+        snipers ^= sb;
+        int sq = lsb(sb);
+        bool inFile = (mFile & sb) != 0, inRank = (mRank & sb) != 0, inDiag = (mDiag & sb) != 0;
+        u64 line = inFile ? mFile : (inRank ? mRank : (inDiag ? mDiag : mAnti));
+        u64 btw = line & (sq > k ? ((1ULL << sq) - (2ULL << k)) : (f.kBit - (2ULL << sq)));
+        u64 blk = btw & s.occ;
+        u64 ray = btw | sb;
+        bool check = blk == 0;
+        bool pin = blk && !(blk & (blk - 1)) && (blk & s.us);
+        chkRays |= check ? ray : 0;
+        behind |= check ? (kAtt & line & (sq > k ? below : above)) : 0;
+        u64 pr = pin ? ray : 0;
+        f.pinFile |= inFile ? pr : 0;
+        f.pinRank |= inRank ? pr : 0;
+        f.pinDiag |= inDiag ? pr : 0;
+        f.pinAnti |= (!inFile && !inRank && !inDiag) ? pr : 0;
+    }
     u64 leaperCheckers = (knightAttacks(f.kBit) & s.knights & s.them) | (pawnAttacks<C>(f.kBit) & s.pawns & s.them);
     f.checkers = leaperCheckers | (chkRays & (eOrth | eDiag));
     f.nCheck = popc(f.checkers);
     f.checkMask = f.nCheck == 0 ? ~0ULL : (f.nCheck == 1 ? (chkRays | leaperCheckers) : 0ULL);
     f.pinned = (f.pinFile | f.pinRank | f.pinDiag | f.pinAnti) & s.us;
+    // leaper and king attacks here; slider attacks are added by the caller
+    f.att = pawnAttacks<1 - C>(s.pawns & s.them) | knightAttacks(s.knights & s.them) | kingAttacks(s.kings & s.them) |
+            behind;
     return f;
+}
+
+template <int C>
+PG_HD Frame frame(const QuadBitBoard &q)
+{
+    Frame f = framePins<C>(q);
+    u64 empty = ~f.s.occ;
+    f.att |= orthAttacksSet(f.s.orth & f.s.them, empty) | diagAttacksSet(f.s.diag & f.s.them, empty);
+    return f;
+}
+
+// One direction of Kogge-Stone fills for two slider sets over the same empty
+// squares: the propagator chain (empty & wrap, shifted and-ed twice) is
+// computed once and shared by the enemy-attack fill and our move fill.
+template <int S, u64 WRAP>
+PG_HD void rayPair(u64 enemy, u64 own, u64 empty, u64 &att, u64 &ownAtt)
+{
+    u64 p0 = empty & WRAP;
+    u64 p1 = p0 & shift<S>(p0);
+    u64 p2 = p1 & shift<2 * S>(p1);
+    enemy |= p0 & shift<S>(enemy);
+    own |= p0 & shift<S>(own);
+    enemy |= p1 & shift<2 * S>(enemy);
+    own |= p1 & shift<2 * S>(own);
+    enemy |= p2 & shift<4 * S>(enemy);
+    own |= p2 & shift<4 * S>(own);
+    att |= shift<S>(enemy) & WRAP;
+    ownAtt = shift<S>(own) & WRAP;
 }
 
 // En-passant moves, reference semantics (MoveGeneratorBitboard.h:1732-1770,
@@ -287,16 +328,45 @@ PG_HD u64 epSources(const Frame &f, u8 state)
 }
 
 // ------------------------------------------------------------- move counting
-// Number of legal moves (perft(1)).  Set-wise, no per-piece loops.
+// Number of legal moves (perft(1)).  Set-wise, no per-piece loops: the eight
+// slider directions are filled once for the enemy's attacks and our moves
+// together; knights and pawns are counted by shifted sets.
 template <int C>
 PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
 {
-    Frame f = frame<C>(q);
+    Frame f = framePins<C>(q);
     const Sets &s = f.s;
     u64 empty = ~s.occ;
     u64 targets = ~s.us & f.checkMask;
-    u32 n = popc(kingAttacks(f.kBit) & ~s.us & ~f.att);
     u64 freeP = ~f.pinned;
+
+    // sliders: a direction's fill reaches each target from exactly one slider,
+    // so per-direction popcounts count moves exactly; a pinned slider moves
+    // only along its pin line
+    u64 D = s.diag & s.us, O = s.orth & s.us;
+    u64 eD = s.diag & s.them, eO = s.orth & s.them;
+    u64 dA1H8 = D & (freeP | f.pinDiag), dA8H1 = D & (freeP | f.pinAnti);
+    u64 oFile = O & (freeP | f.pinFile), oRank = O & (freeP | f.pinRank);
+    u64 att = f.att, r;
+    u32 n = 0;
+    rayPair<8, ~0ULL>(eO, oFile, empty, att, r);
+    n += popc(r & targets);
+    rayPair<-8, ~0ULL>(eO, oFile, empty, att, r);
+    n += popc(r & targets);
+    rayPair<1, NOT_A>(eO, oRank, empty, att, r);
+    n += popc(r & targets);
+    rayPair<-1, NOT_H>(eO, oRank, empty, att, r);
+    n += popc(r & targets);
+    rayPair<9, NOT_A>(eD, dA1H8, empty, att, r);
+    n += popc(r & targets);
+    rayPair<-9, NOT_H>(eD, dA1H8, empty, att, r);
+    n += popc(r & targets);
+    rayPair<7, NOT_H>(eD, dA8H1, empty, att, r);
+    n += popc(r & targets);
+    rayPair<-7, NOT_A>(eD, dA8H1, empty, att, r);
+    n += popc(r & targets);
+
+    n += popc(kingAttacks(f.kBit) & ~s.us & ~att);
 
     // knights: pinned knights never move
     u64 N = s.knights & s.us & freeP;
@@ -307,16 +377,6 @@ PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
              popc((r1 >> 16) & targets) + popc((l2 << 8) & targets) + popc((l2 >> 8) & targets) +
              popc((r2 << 8) & targets) + popc((r2 >> 8) & targets);
     }
-
-    // sliders: a direction's fill reaches each target from exactly one slider,
-    // so per-direction popcounts count moves exactly
-    u64 D = s.diag & s.us, O = s.orth & s.us;
-    u64 dA1H8 = D & (freeP | f.pinDiag), dA8H1 = D & (freeP | f.pinAnti);
-    u64 oFile = O & (freeP | f.pinFile), oRank = O & (freeP | f.pinRank);
-    n += popc(rayAttacks<9, NOT_A>(dA1H8, empty) & targets) + popc(rayAttacks<-9, NOT_H>(dA1H8, empty) & targets) +
-         popc(rayAttacks<7, NOT_H>(dA8H1, empty) & targets) + popc(rayAttacks<-7, NOT_A>(dA8H1, empty) & targets) +
-         popc(rayAttacks<8, ~0ULL>(oFile, empty) & targets) + popc(rayAttacks<-8, ~0ULL>(oFile, empty) & targets) +
-         popc(rayAttacks<1, NOT_A>(oRank, empty) & targets) + popc(rayAttacks<-1, NOT_H>(oRank, empty) & targets);
 
     // pawns
     u64 P = s.pawns & s.us;
@@ -339,8 +399,8 @@ PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
     const u64 qsEmpty = C == WHITE ? 0x0EULL : 0x0E00000000000000ULL;
     const u64 qsSafe = C == WHITE ? 0x0CULL : 0x0C00000000000000ULL;
     bool noCheck = f.nCheck == 0;
-    n += (noCheck && (cr & 1) && !((s.occ | f.att) & ksEmpty)) ? 1 : 0;
-    n += (noCheck && (cr & 2) && !(s.occ & qsEmpty) && !(f.att & qsSafe)) ? 1 : 0;
+    n += (noCheck && (cr & 1) && !((s.occ | att) & ksEmpty)) ? 1 : 0;
+    n += (noCheck && (cr & 2) && !(s.occ & qsEmpty) && !(att & qsSafe)) ? 1 : 0;
 
     if (state >> 4) n += popc(epSources<C>(f, state));
     return n;

@@ -133,3 +133,19 @@ def test_serving_alone_equals_in_mix(gpu):
     assert mixed == want
     for rid in list(reqs)[:10]:  # each alone, one slot, nothing else in flight
         assert _serve(gpu, {rid: reqs[rid]}, slots=1) == {rid: want[rid]}
+
+
+@pytest.mark.parametrize("leaf", [1, 2])
+def test_batch_with_tt_matches_golden(gpu, leaf):
+    gpu.tt_init(1024, 0)
+    try:
+        rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] <= 5][:800]
+        ps = [gpu.parse_fen(r["fen"]) for r in rows]
+        ds = [r["depth"] for r in rows]
+        want = [r["count"] for r in rows]
+        for rep in range(2):  # the second pass is served mostly by the table
+            assert gpu.perft_batch(ps, ds, leaf_plies=leaf, use_tt=True) == want
+        st = gpu.tt_stats()
+        assert st[1] > 0 and st[2] > 0, st
+    finally:
+        gpu.tt_free()

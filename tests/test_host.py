@@ -139,3 +139,34 @@ def test_u128_printing(engine):
     for v in [0, 1, 2**64 - 1, 2**64, 61885021521585529237, 2**128 - 1, 10**30]:
         engine.lib().pg_u128_to_string(v & (2**64 - 1), v >> 64, buf, 64)
         assert buf.value.decode() == str(v)
+
+
+def test_cli_divide_cpu(engine, orc):
+    fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    r = engine.run_cli([fen, "3", "-cpu", "-nott", "-divide"])
+    assert r.returncode == 0, r.stdout
+    lines = [l for l in r.stdout.splitlines() if len(l) > 4 and l[4] in ":qrbn"]
+    assert len(lines) == 48
+    assert "Total: 97862" in r.stdout
+    # each root move's count equals the oracle's perft(2) of the child
+    root = orc.parse_fen(fen)
+    by_text = {}
+    for m in orc.gen_moves(root):
+        t = "abcdefgh"[m & 7] + str(((m >> 3) & 7) + 1) + "abcdefgh"[(m >> 6) & 7] + str(((m >> 9) & 7) + 1)
+        if (m >> 12) >= 8:
+            t += "nbrq"[(m >> 12) & 3]
+        by_text[t] = orc.perft(orc.make_move(root, m), 2)
+    for l in lines:
+        mv, n = l.split(":")
+        assert by_text[mv.strip()] == int(n)
+
+
+def test_cli_suite_cpu(engine, tmp_path):
+    f = tmp_path / "suite.epd"
+    f.write_text("rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1 ;D1 20 ;D2 400 ;D3 8902\n"
+                 "8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - ;D1 14 ;D2 191 ;D3 2812 ;D4 43238\n")
+    r = engine.run_cli(["-suite", str(f), "4", "-cpu"])
+    assert r.returncode == 0 and "7 checks, 0 failures" in r.stdout, r.stdout
+    f.write_text("rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1 ;D1 21\n")
+    r = engine.run_cli(["-suite", str(f), "4", "-cpu"])
+    assert r.returncode == 1 and "1 failures" in r.stdout
