@@ -1,6 +1,7 @@
 """Benchmark: perft nodes/s on one MI355X.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--leaf 1|2] [--batch]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--leaf 1|2]
+                  [--batch] [--serve N] [--serve-rate R]
 
 Workload (one step): perft(7) of the config-2 position -- the start position
 after 8 seeded random plies (seed 2, SplitMix64 walk over canonically sorted
@@ -20,7 +21,11 @@ The JSON line carries:
   cpu_baseline  the oracle (a C restatement of the reference's single-threaded
                 -cpu path) on this host, one core, on a bounded sample of the
                 same tree: perft(6) under the first root moves.
-With --batch it also times the 4096-position config-3 batch, batched vs plain.
+With --batch it also times the 4096-position config-3 batch, batched vs plain
+(both checked against the golden counts).  With --serve N it replays the first
+N config-5 requests through the serving loop with Poisson arrivals (rate R
+requests/s; default 0.75 x the capacity measured by a closed-loop pass) and
+reports p50/p99 latency and nodes/s.
 """
 from __future__ import annotations
 
@@ -69,6 +74,93 @@ def roofline(stat: dict) -> dict:
             "algorithmic_bytes_per_launch": round(bytes_per_launch)}
 
 
+def batch_leg(pg) -> dict:
+    rows = load_fixture("c3.jsonl")
+    ps = [pg.parse_fen(r["fen"]) for r in rows]
+    ds = [r["depth"] for r in rows]
+    want = [r["count"] for r in rows]
+    total = sum(want)
+    got = pg.perft_batch(ps, ds)  # warm: autotune + graph capture
+    assert got == want
+    t0 = time.perf_counter()
+    got = pg.perft_batch(ps, ds)
+    t_b = time.perf_counter() - t0
+    assert got == want
+    pg.perft_batch_plain(ps[:64], ds[:64])
+    t0 = time.perf_counter()
+    got = pg.perft_batch_plain(ps, ds)
+    t_plain = time.perf_counter() - t0
+    assert got == want
+    return {"workload": "config-3: 4096 positions, 6-40 random plies, depth 3-6, no TT", "nodes": total,
+            "batched_s": t_b, "batched_nodes_per_s": total / t_b, "plain_s": t_plain,
+            "plain_nodes_per_s": total / t_plain, "batched_over_plain": t_plain / t_b}
+
+
+def serve_leg(pg, n_req: int, rate: float) -> dict:
+    import ctypes
+    import random
+    import threading
+
+    rows = load_fixture("c5.jsonl")[:n_req]
+    reqs = [(r["seed"], pg.parse_fen(r["fen"]), r["depth"]) for r in rows]
+    gold = {r["seed"]: r["count"] for r in rows if "count" in r}
+    L = pg.lib()
+
+    def run(arrivals):
+        cfg = pg.ServeConfig(200 << 30, 16 << 30, 8, 1)
+        h = ctypes.c_void_p()
+        if L.pg_serve_create(ctypes.byref(cfg), ctypes.byref(h)) != 0:
+            raise RuntimeError(pg.last_error())
+        done = {}
+        stop = threading.Event()
+
+        def poller():
+            res = (pg.ServeResult * 256)()
+            n = ctypes.c_int()
+            while not stop.is_set() or len(done) < len(reqs):
+                L.pg_serve_poll(h, res, 256, 50, ctypes.byref(n))
+                for i in range(n.value):
+                    r = res[i]
+                    done[r.id] = (r.count, r.submitTime, r.startTime, r.endTime, r.status)
+                if stop.is_set() and len(done) >= len(reqs):
+                    break
+
+        th = threading.Thread(target=poller)
+        th.start()
+        t0 = L.pg_clock()
+        for (rid, p, d), at in zip(reqs, arrivals):
+            delay = t0 + at - L.pg_clock()
+            if delay > 0:
+                time.sleep(delay)
+            L.pg_serve_submit(h, rid, ctypes.byref(p), d)
+        stop.set()
+        th.join()
+        L.pg_serve_destroy(h)
+        return done
+
+    # closed loop (everything at t=0) measures capacity
+    done = run([0.0] * len(reqs))
+    first = min(v[1] for v in done.values())
+    last = max(v[3] for v in done.values())
+    capacity = len(reqs) / (last - first)
+    lam = rate or 0.75 * capacity
+    rnd = random.Random(5)
+    t, arrivals = 0.0, []
+    for _ in reqs:
+        t += rnd.expovariate(lam)
+        arrivals.append(t)
+    done = run(arrivals)
+    bad = [k for k, v in done.items() if v[4] != 0 or (k in gold and v[0] != gold[k])]
+    lat = sorted(v[3] - v[1] for v in done.values())
+    nodes = sum(v[0] for v in done.values())
+    span = max(v[3] for v in done.values()) - min(v[1] for v in done.values())
+    return {"workload": f"config-5: first {len(reqs)} requests (0-60 plies, depth 3-7), TT shared, "
+                        "200 GB budget, 8 slots, Poisson arrivals",
+            "rate_req_per_s": lam, "capacity_req_per_s": capacity, "p50_ms": 1e3 * lat[len(lat) // 2],
+            "p99_ms": 1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], "nodes_per_s": nodes / span,
+            "golden_checked": sum(1 for k in done if k in gold), "mismatches": len(bad)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +169,8 @@ def main():
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--serve", type=int, default=0)
+    ap.add_argument("--serve-rate", type=float, default=0.0)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -138,24 +232,11 @@ def main():
                     for k, v in stats.items()},
     }
     if a.batch:
-        rows = load_fixture("c3.jsonl")
-        ps = [pg.parse_fen(r["fen"]) for r in rows]
-        ds = [r["depth"] for r in rows]
-        total = sum(r["count"] for r in rows)
-        t0 = time.perf_counter()
-        got = pg.perft_batch_plain(ps, ds)
-        t_plain = time.perf_counter() - t0
-        assert got == [r["count"] for r in rows]
-        res = {"plain_nodes_per_s": total / t_plain, "plain_s": t_plain, "nodes": total}
-        try:
-            t0 = time.perf_counter()
-            got = pg.perft_batch(ps, ds)
-            t_b = time.perf_counter() - t0
-            assert got == [r["count"] for r in rows]
-            res.update({"batched_nodes_per_s": total / t_b, "batched_s": t_b, "ratio": t_plain / t_b})
-        except pg.PerftError as e:
-            res["batched_error"] = str(e)
-        out["batch_config3"] = res
+        print("[bench] config-3 batch", file=sys.stderr, flush=True)
+        out["batch_config3"] = batch_leg(pg)
+    if a.serve:
+        print("[bench] config-5 serving", file=sys.stderr, flush=True)
+        out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate)
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(fx["fen"], depth)

@@ -147,10 +147,22 @@ PG_HD u64 rayAttacks(u64 gen, u64 empty)
     return shift<S>(gen) & WRAP;
 }
 
+// East attacks of every slider in s, rank-wise subtraction (o ^ (o - 2s)):
+// the h-file is treated as occupied so a borrow never leaves its rank, and
+// h-file sliders (no east moves) are dropped.
+PG_HD u64 eastAttacks(u64 s, u64 occ)
+{
+    u64 o = occ | FILE_H;
+    u64 g = s & NOT_H;
+    return o ^ (o - (g << 1));
+}
+// West attacks: east attacks in the bit-reversed board
+PG_HD u64 westAttacks(u64 s, u64 occ) { return rev64(eastAttacks(rev64(s), rev64(occ))); }
+
 PG_HD u64 orthAttacksSet(u64 s, u64 empty)
 {
-    return rayAttacks<8, ~0ULL>(s, empty) | rayAttacks<-8, ~0ULL>(s, empty) |
-           rayAttacks<1, NOT_A>(s, empty) | rayAttacks<-1, NOT_H>(s, empty);
+    return rayAttacks<8, ~0ULL>(s, empty) | rayAttacks<-8, ~0ULL>(s, empty) | eastAttacks(s, ~empty) |
+           westAttacks(s, ~empty);
 }
 PG_HD u64 diagAttacksSet(u64 s, u64 empty)
 {
@@ -232,27 +244,27 @@ PG_HD Frame framePins(const QuadBitBoard &q)
     u64 snipers = ((mFile | mRank) & eOrth) | ((mDiag | mAnti) & eDiag);
     u64 below = f.kBit - 1, above = ~below & ~f.kBit;
     u64 kAtt = kingAttacks(f.kBit);
-    u64 chkRays = 0, behind = 0;
-    f.pinFile = f.pinRank = f.pinDiag = f.pinAnti = 0;
+    u64 chkRays = 0, behind = 0, pinAll = 0;
     while (snipers) {
         u64 sb = snipers & (0 - snipers);
         snipers ^= sb;
         int sq = lsb(sb);
-        bool inFile = (mFile & sb) != 0, inRank = (mRank & sb) != 0, inDiag = (mDiag & sb) != 0;
-        u64 line = inFile ? mFile : (inRank ? mRank : (inDiag ? mDiag : mAnti));
-        u64 btw = line & (sq > k ? ((1ULL << sq) - (2ULL << k)) : (f.kBit - (2ULL << sq)));
+        u64 line = (mFile & sb) ? mFile : ((mRank & sb) ? mRank : ((mDiag & sb) ? mDiag : mAnti));
+        bool up = sq > k;
+        u64 btw = line & (up ? (sb - (f.kBit << 1)) : (f.kBit - (sb << 1)));
         u64 blk = btw & s.occ;
         u64 ray = btw | sb;
         bool check = blk == 0;
         bool pin = blk && !(blk & (blk - 1)) && (blk & s.us);
         chkRays |= check ? ray : 0;
-        behind |= check ? (kAtt & line & (sq > k ? below : above)) : 0;
-        u64 pr = pin ? ray : 0;
-        f.pinFile |= inFile ? pr : 0;
-        f.pinRank |= inRank ? pr : 0;
-        f.pinDiag |= inDiag ? pr : 0;
-        f.pinAnti |= (!inFile && !inRank && !inDiag) ? pr : 0;
+        behind |= check ? (kAtt & line & (up ? below : above)) : 0;
+        pinAll |= pin ? ray : 0;
     }
+    // pin rays of different lines meet only at the king, so the lines split them
+    f.pinFile = pinAll & mFile;
+    f.pinRank = pinAll & mRank;
+    f.pinDiag = pinAll & mDiag;
+    f.pinAnti = pinAll & mAnti;
     u64 leaperCheckers = (knightAttacks(f.kBit) & s.knights & s.them) | (pawnAttacks<C>(f.kBit) & s.pawns & s.them);
     f.checkers = leaperCheckers | (chkRays & (eOrth | eDiag));
     f.nCheck = popc(f.checkers);
@@ -353,10 +365,8 @@ PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
     n += popc(r & targets);
     rayPair<-8, ~0ULL>(eO, oFile, empty, att, r);
     n += popc(r & targets);
-    rayPair<1, NOT_A>(eO, oRank, empty, att, r);
-    n += popc(r & targets);
-    rayPair<-1, NOT_H>(eO, oRank, empty, att, r);
-    n += popc(r & targets);
+    att |= eastAttacks(eO, s.occ) | westAttacks(eO, s.occ);
+    n += popc(eastAttacks(oRank, s.occ) & targets) + popc(westAttacks(oRank, s.occ) & targets);
     rayPair<9, NOT_A>(eD, dA1H8, empty, att, r);
     n += popc(r & targets);
     rayPair<-9, NOT_H>(eD, dA1H8, empty, att, r);
