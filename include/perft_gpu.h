@@ -130,6 +130,11 @@ int pg_tt_clear(void);          /* clearDeviceTTs, launcher.cu:489 */
 int pg_tt_free(void);           /* freeTT */
 /* printTTStats-style counters: device probes/hits/stores, host probes/hits/stores */
 int pg_tt_stats(uint64_t out[6]);
+/* TT snapshot / resume (no reference counterpart; its 27.8 h perft 14 run had
+ * no resume, README.md:63): write both tiers to a file, or load a file into
+ * tables created by pg_tt_init with the same sizes */
+int pg_tt_save(const char *path);
+int pg_tt_load(const char *path);
 
 /* ===================== kernel-level ABI (device pointers) =================== */
 /* One BFS level of the reference's host-driven loop (perft_kernels.cu:681-835),

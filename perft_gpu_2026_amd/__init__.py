@@ -105,6 +105,8 @@ _SIGS = [
     ("pg_tt_clear", c_int, []),
     ("pg_tt_free", c_int, []),
     ("pg_tt_stats", c_int, [POINTER(c_uint64)]),
+    ("pg_tt_save", c_int, [c_char_p]),
+    ("pg_tt_load", c_int, [c_char_p]),
     ("pg_level_makecount", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uint64, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p]),
     ("pg_scan_u32", c_int, [c_void_p, c_uint64, c_void_p, c_void_p, POINTER(c_uint64), c_void_p]),

@@ -274,16 +274,15 @@ __device__ __forceinline__ void ttCount(u64 *ctr, bool probed, bool hit, bool st
     }
 }
 
+// make + table probe (TT mode): a hit is a known subtree
 template <int C>
-__global__ __launch_bounds__(BS) void k_makecount_tt(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
-                                                     const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
-                                                     QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
-                                                     u32 *__restrict__ cnt, u32 *__restrict__ tileSums,
-                                                     u64 *__restrict__ known, const TTEntry *__restrict__ tt,
-                                                     u64 mask, int rd, u64 *ctr)
+__global__ __launch_bounds__(BS) void k_make_probe(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
+                                                   const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
+                                                   QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
+                                                   u64 *__restrict__ known, const TTEntry *__restrict__ tt, u64 mask,
+                                                   int rd, u64 *ctr)
 {
     u64 i = (u64)blockIdx.x * BS + threadIdx.x;
-    u32 c = 0;
     bool hit = false;
     if (i < n) {
         u32 p = pidx[i];
@@ -295,12 +294,65 @@ __global__ __launch_bounds__(BS) void k_makecount_tt(const QuadBitBoard *__restr
         u64 v;
         hit = ttProbe(tt, mask, hashPosition(q, st, 1 - C), rd, &v);
         known[i] = hit ? v + 1 : 0;
-        c = hit ? 0 : countMoves<1 - C>(q, st);
-        cnt[i] = c;
     }
     ttCount(ctr, i < n, hit, false);
-    u64 s = blockSum64(c);
-    if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)s;
+}
+
+// Within-level deduplication (write_dedup_table / check_duplicates /
+// apply_redirects, perft_kernels.cu:486-552): every unknown entry writes
+// {hash.hi, index+1} to a cleared table, then reads its slot back; an entry
+// that finds another index with its key AND an identical position (boards
+// compared, so a hash collision or a torn 16-B slot can never merge two
+// different positions) becomes a duplicate with no children, and takes the
+// original's value after the upsweep.
+struct DedupEntry {
+    u64 key, idx;
+};
+
+template <int C>  // side to move in these boards
+__global__ __launch_bounds__(BS) void k_dedup_write(const QuadBitBoard *__restrict__ b, const u8 *__restrict__ s,
+                                                    const u64 *__restrict__ known, u64 n, DedupEntry *__restrict__ tab,
+                                                    u64 tmask)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    if (i >= n || known[i]) return;
+    Hash128 h = hashPosition(b[i], s[i], C);
+    tab[(h.lo >> 17) & tmask] = DedupEntry{h.hi, i + 1};
+}
+
+template <int C>
+__global__ __launch_bounds__(BS) void k_dedup_count(const QuadBitBoard *__restrict__ b, const u8 *__restrict__ s,
+                                                    const u64 *__restrict__ known, u64 n,
+                                                    const DedupEntry *__restrict__ tab, u64 tmask,
+                                                    u32 *__restrict__ redir, u32 *__restrict__ cnt,
+                                                    u32 *__restrict__ tileSums)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    u32 c = 0;
+    if (i < n) {
+        u32 r = (u32)i;
+        if (!known[i]) {
+            QuadBitBoard q = b[i];
+            u8 st = s[i];
+            Hash128 h = hashPosition(q, st, C);
+            DedupEntry e = tab[(h.lo >> 17) & tmask];
+            u64 j = e.idx - 1;
+            bool dup = e.key == h.hi && e.idx && j != i && j < n && s[j] == st && b[j].bb[0] == q.bb[0] &&
+                       b[j].bb[1] == q.bb[1] && b[j].bb[2] == q.bb[2] && b[j].bb[3] == q.bb[3];
+            if (dup) r = (u32)j;
+            else c = countMoves<C>(q, st);
+        }
+        redir[i] = r;
+        cnt[i] = c;
+    }
+    u64 t = blockSum64(c);
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)t;
+}
+
+__global__ __launch_bounds__(BS) void k_apply_redirects(const u32 *__restrict__ redir, u64 n, u64 *__restrict__ val)
+{
+    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
+    if (i < n && redir[i] != (u32)i) val[i] = val[redir[i]];
 }
 
 // one-ply leaf with per-entry values
@@ -394,12 +446,13 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard 
 template <int C>  // side to move in this level's boards
 __global__ __launch_bounds__(BS) void k_upsweep(const QuadBitBoard *__restrict__ boards, const u8 *__restrict__ states,
                                                 const u32 *__restrict__ incl, const u64 *__restrict__ known,
-                                                const u64 *__restrict__ childVal, u64 n, u64 *__restrict__ val,
-                                                TTEntry *__restrict__ tt, u64 mask, int rd, u64 *ctr)
+                                                const u32 *__restrict__ redir, const u64 *__restrict__ childVal,
+                                                u64 n, u64 *__restrict__ val, TTEntry *__restrict__ tt, u64 mask,
+                                                int rd, u64 *ctr)
 {
     u64 i = (u64)blockIdx.x * BS + threadIdx.x;
     bool store = false;
-    if (i < n) {
+    if (i < n && redir[i] == (u32)i) {  // duplicates are filled in afterwards
         u64 v;
         if (known[i]) {
             v = known[i] - 1;
@@ -503,6 +556,7 @@ struct TTLevel {
     const u8 *states;
     u32 *incl;
     u64 *known;
+    u32 *redir;
     u64 *val;
     u64 n;
 };
@@ -547,20 +601,31 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         u64 *val = A.alloc<u64>(n);
         u32 *tileSums = A.alloc<u32>(tiles);
         u64 *tileOffs = A.alloc<u64>(tiles + 1);
-        if (!ob || !os || !cnt || !known || !val || !tileSums || !tileOffs) return PG_ERR_ARENA_OOM;
+        u32 *redir = A.alloc<u32>(n);
+        u64 tslots = 1;
+        while (tslots < 2 * n) tslots <<= 1;
+        DedupEntry *tab = A.alloc<DedupEntry>(tslots);
+        if (!ob || !os || !cnt || !known || !val || !tileSums || !tileOffs || !redir || !tab) return PG_ERR_ARENA_OOM;
         dim3 g(gridFor(n, BS)), b(BS);
         int rd = depth - lvl;
-        if (pc == WHITE)
-            hipLaunchKernelGGL(k_makecount_tt<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, cnt, tileSums, known, tt,
-                               mask, rd, ctr);
-        else
-            hipLaunchKernelGGL(k_makecount_tt<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, cnt, tileSums, known, tt,
-                               mask, rd, ctr);
+        PG_HIP(hipMemsetAsync(tab, 0, tslots * sizeof(DedupEntry), st));
+        if (pc == WHITE) {
+            hipLaunchKernelGGL(k_make_probe<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
+            hipLaunchKernelGGL(k_dedup_write<BLACK>, g, b, 0, st, ob, os, known, n, tab, tslots - 1);
+            hipLaunchKernelGGL(k_dedup_count<BLACK>, g, b, 0, st, ob, os, known, n, tab, tslots - 1, redir, cnt,
+                               tileSums);
+        } else {
+            hipLaunchKernelGGL(k_make_probe<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
+            hipLaunchKernelGGL(k_dedup_write<WHITE>, g, b, 0, st, ob, os, known, n, tab, tslots - 1);
+            hipLaunchKernelGGL(k_dedup_count<WHITE>, g, b, 0, st, ob, os, known, n, tab, tslots - 1, redir, cnt,
+                               tileSums);
+        }
+        PG_HIP(hipGetLastError());
         PG_TRY(launchScan(cnt, n, tileSums, tileOffs, st));
         PG_HIP(hipMemcpyAsync(pinned, tileOffs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         u64 c = pinned[0];
-        lv.push_back({ob, os, cnt, known, val, n});
+        lv.push_back({ob, os, cnt, known, redir, val, n});
         if (c >= 0xFFFFFFFFull) return PG_ERR_ARENA_OOM;
         u32 *parts = A.alloc<u32>((c + EXPAND_TILE - 1) / EXPAND_TILE + 1);
         u32 *nidx = A.alloc<u32>(c);
@@ -602,11 +667,12 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         int side = color ^ (lvl & 1);
         dim3 g(gridFor(l.n, BS)), b(BS);
         if (side == WHITE)
-            hipLaunchKernelGGL(k_upsweep<WHITE>, g, b, 0, st, l.boards, l.states, l.incl, l.known, childVal, l.n, l.val,
-                               tt, mask, depth - lvl, ctr);
+            hipLaunchKernelGGL(k_upsweep<WHITE>, g, b, 0, st, l.boards, l.states, l.incl, l.known, l.redir, childVal,
+                               l.n, l.val, tt, mask, depth - lvl, ctr);
         else
-            hipLaunchKernelGGL(k_upsweep<BLACK>, g, b, 0, st, l.boards, l.states, l.incl, l.known, childVal, l.n, l.val,
-                               tt, mask, depth - lvl, ctr);
+            hipLaunchKernelGGL(k_upsweep<BLACK>, g, b, 0, st, l.boards, l.states, l.incl, l.known, l.redir, childVal,
+                               l.n, l.val, tt, mask, depth - lvl, ctr);
+        hipLaunchKernelGGL(k_apply_redirects, g, b, 0, st, l.redir, l.n, l.val);
         childVal = l.val;
     }
     hipLaunchKernelGGL(k_sum_u64, dim3(gridFor(n1, BS)), dim3(BS), 0, st, childVal, (u64)n1, dSum);

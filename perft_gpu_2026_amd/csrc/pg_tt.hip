@@ -4,6 +4,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <vector>
 
@@ -172,6 +173,92 @@ int ttStats(uint64_t out[6])
     return PG_OK;
 }
 
+// ----------------------------------------------------------- snapshot/resume
+// File: magic "PGTT0001", device slot count, host bucket count, host pool
+// entries in use, then the device table, the host heads and the host pool.
+// A long run can save its tables and a later process resume from them.
+static const char TT_MAGIC[8] = {'P', 'G', 'T', 'T', '0', '0', '0', '1'};
+
+int ttSave(const char *path)
+{
+    Lib &L = lib();
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        setError("cannot open %s for writing", path);
+        return PG_ERR_INVALID;
+    }
+    u64 devSlots = L.tt.entries ? L.tt.mask + 1 : 0;
+    u64 buckets = g_host.heads ? g_host.mask + 1 : 0;
+    u64 used = g_host.heads ? std::min<u64>(g_host.used.load(), g_host.poolSize) : 0;
+    u64 hdr[3] = {devSlots, buckets, used};
+    bool ok = fwrite(TT_MAGIC, 1, 8, f) == 8 && fwrite(hdr, 8, 3, f) == 3;
+    const u64 CH = 64ull << 20;
+    std::vector<char> buf;
+    if (ok && devSlots) {
+        buf.resize(CH);
+        for (u64 off = 0; ok && off < devSlots * 16; off += CH) {
+            u64 len = std::min(CH, devSlots * 16 - off);
+            if (hipMemcpy(buf.data(), (char *)L.tt.entries + off, len, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+            else ok = fwrite(buf.data(), 1, len, f) == len;
+        }
+    }
+    if (ok && buckets) ok = fwrite((const void *)g_host.heads, sizeof(u32), buckets, f) == buckets;
+    if (ok && used) ok = fwrite(g_host.pool, sizeof(HostEntry), used, f) == used;
+    fclose(f);
+    if (!ok) {
+        setError("writing %s failed", path);
+        return PG_ERR_INVALID;
+    }
+    return PG_OK;
+}
+
+int ttLoad(const char *path)
+{
+    Lib &L = lib();
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        setError("cannot open %s", path);
+        return PG_ERR_INVALID;
+    }
+    char magic[8];
+    u64 hdr[3];
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, TT_MAGIC, 8) || fread(hdr, 8, 3, f) != 3) {
+        fclose(f);
+        setError("%s is not a TT snapshot", path);
+        return PG_ERR_INVALID;
+    }
+    u64 devSlots = hdr[0], buckets = hdr[1], used = hdr[2];
+    // the tables must already exist with the same geometry (pg_tt_init with the same sizes)
+    u64 haveDev = L.tt.entries ? L.tt.mask + 1 : 0, haveBuckets = g_host.heads ? g_host.mask + 1 : 0;
+    if (devSlots != haveDev || buckets != haveBuckets || used > g_host.poolSize) {
+        fclose(f);
+        setError("snapshot geometry (%llu slots, %llu buckets) differs from the current tables (%llu, %llu)",
+                 (unsigned long long)devSlots, (unsigned long long)buckets, (unsigned long long)haveDev,
+                 (unsigned long long)haveBuckets);
+        return PG_ERR_INVALID;
+    }
+    bool ok = true;
+    const u64 CH = 64ull << 20;
+    std::vector<char> buf;
+    if (devSlots) {
+        buf.resize(CH);
+        for (u64 off = 0; ok && off < devSlots * 16; off += CH) {
+            u64 len = std::min(CH, devSlots * 16 - off);
+            ok = fread(buf.data(), 1, len, f) == len &&
+                 hipMemcpy((char *)L.tt.entries + off, buf.data(), len, hipMemcpyHostToDevice) == hipSuccess;
+        }
+    }
+    if (ok && buckets) ok = fread((void *)g_host.heads, sizeof(u32), buckets, f) == buckets;
+    if (ok && used) ok = fread(g_host.pool, sizeof(HostEntry), used, f) == used;
+    if (ok && buckets) g_host.used = used ? used : 1;
+    fclose(f);
+    if (!ok) {
+        setError("reading %s failed", path);
+        return PG_ERR_INVALID;
+    }
+    return PG_OK;
+}
+
 void u128::toDecimal(char *buf, int n) const
 {
     if (hi == 0) {
@@ -209,6 +296,16 @@ int pg_tt_stats(uint64_t out[6])
 {
     if (!out) return PG_ERR_INVALID;
     return ttStats(out);
+}
+int pg_tt_save(const char *path)
+{
+    if (!path) return PG_ERR_INVALID;
+    return ttSave(path);
+}
+int pg_tt_load(const char *path)
+{
+    if (!path) return PG_ERR_INVALID;
+    return ttLoad(path);
 }
 int pg_u128_to_string(uint64_t lo, uint64_t hi, char *buf, int n)
 {

@@ -149,3 +149,23 @@ def test_batch_with_tt_matches_golden(gpu, leaf):
         assert st[1] > 0 and st[2] > 0, st
     finally:
         gpu.tt_free()
+
+
+def test_tt_snapshot_resume_device(gpu, tmp_path):
+    p = gpu.parse_fen(KIWI)
+    gpu.tt_init(256, 16)
+    try:
+        assert gpu.perft(p, 5, launch_depth=4, use_tt=True) == 193690690
+        snap = str(tmp_path / "tt.bin").encode()
+        assert gpu.lib().pg_tt_save(snap) == 0
+    finally:
+        gpu.tt_free()
+    gpu.tt_init(256, 16)
+    try:
+        assert gpu.lib().pg_tt_load(snap) == 0
+        before = gpu.tt_stats()
+        assert gpu.perft(p, 5, launch_depth=4, use_tt=True) == 193690690
+        after = gpu.tt_stats()
+        assert after[4] > before[4]  # host tier answers from the snapshot
+    finally:
+        gpu.tt_free()

@@ -170,3 +170,29 @@ def test_cli_suite_cpu(engine, tmp_path):
     f.write_text("rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1 ;D1 21\n")
     r = engine.run_cli(["-suite", str(f), "4", "-cpu"])
     assert r.returncode == 1 and "1 failures" in r.stdout
+
+
+def test_host_tt_snapshot_resume(engine, tmp_path):
+    # host tier only (no GPU here): save after a run, reload into fresh tables,
+    # and the reloaded table answers the same queries exactly
+    fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    p = engine.parse_fen(fen)
+    engine.tt_init(0, 32)
+    try:
+        assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+        snap = str(tmp_path / "tt.bin")
+        assert engine.lib().pg_tt_save(snap.encode()) == 0
+    finally:
+        engine.tt_free()
+    engine.tt_init(0, 32)
+    try:
+        assert engine.lib().pg_tt_load(snap.encode()) == 0
+        before = engine.tt_stats()
+        assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+        after = engine.tt_stats()
+        assert after[4] - before[4] >= 1  # the root itself is a hit after the reload
+        engine.tt_free()
+        engine.tt_init(0, 64)  # different geometry is refused
+        assert engine.lib().pg_tt_load(snap.encode()) == engine.PG_ERR_INVALID
+    finally:
+        engine.tt_free()
