@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "pg_internal.h"
@@ -39,6 +40,8 @@ static std::atomic<uint64_t> g_oomCount{0};
 static size_t g_lastPeak = 0;
 static uint64_t g_arenaLimit = 0;  // 0 = whole arena; a test hook to force OOM
 int g_leafPlies = 1;
+static bool g_frontierBatch = true;  // launcher: batch the frontier above the launch depth
+static const int FRONTIER_MAX_PLIES = 4;
 
 static inline QuadBitBoard toQBB(const pg_position *p)
 {
@@ -291,6 +294,86 @@ int gpuBfs(const pg_position *pos, int depth, bool useTT, void *buf, uint64_t by
 // perft_cpu_recurse (launcher.cu:572-715): host TT probe, GPU call at or
 // below the effective launch depth, CPU split one ply down on OOM (which
 // also lowers the effective launch depth), host TT store of 64-bit results.
+// Host frontier feeding the device (the reference recurses serially above the
+// launch depth and makes one GPU call per node, launcher.cu:572-715): expand
+// `depth - ld` plies on the host, merge identical positions with their
+// multiplicities (exact: the whole position is the key), and run every unique
+// frontier position through the batched engine in one pass.
+struct PosKey {
+    u64 bb[4];
+    u8 state, color;
+    bool operator==(const PosKey &o) const
+    {
+        return bb[0] == o.bb[0] && bb[1] == o.bb[1] && bb[2] == o.bb[2] && bb[3] == o.bb[3] && state == o.state &&
+               color == o.color;
+    }
+};
+struct PosKeyHash {
+    size_t operator()(const PosKey &k) const
+    {
+        u64 h = (u64)k.state | ((u64)k.color << 8);
+        for (int i = 0; i < 4; i++) h = (h ^ k.bb[i]) * 0x9E3779B97F4A7C15ULL + (h >> 29);
+        return (size_t)(h ^ (h >> 31));
+    }
+};
+
+static int frontierPerft(const pg_position *pos, int depth, int ld, bool useTT, u128 *out)
+{
+    std::unordered_map<PosKey, u64, PosKeyHash> cur, next;
+    PosKey k0;
+    memcpy(k0.bb, pos->bb, 32);
+    k0.state = pos->state;
+    k0.color = pos->color;
+    cur[k0] = 1;
+    for (int ply = 0; ply < depth - ld; ply++) {
+        next.clear();
+        next.reserve(cur.size() * 32);
+        for (auto &kv : cur) {
+            QuadBitBoard q;
+            memcpy(q.bb, kv.first.bb, 32);
+            u16 mv[MAX_MOVES];
+            u32 n = generateMoves(q, kv.first.state, kv.first.color, mv);
+            for (u32 i = 0; i < n; i++) {
+                PosKey c;
+                QuadBitBoard cq = q;
+                c.state = kv.first.state;
+                makeMove(cq, c.state, kv.first.color, mv[i]);
+                memcpy(c.bb, cq.bb, 32);
+                c.color = kv.first.color ^ 1;
+                next[c] += kv.second;
+            }
+        }
+        std::swap(cur, next);
+    }
+    std::vector<pg_position> ps;
+    std::vector<u64> mult;
+    ps.reserve(cur.size());
+    mult.reserve(cur.size());
+    for (auto &kv : cur) {
+        pg_position p{};
+        memcpy(p.bb, kv.first.bb, 32);
+        p.state = kv.first.state;
+        p.color = kv.first.color;
+        ps.push_back(p);
+        mult.push_back(kv.second);
+    }
+    std::vector<u8> ds(ps.size(), (u8)ld);
+    std::vector<u64> counts(ps.size());
+    pg_batch_options opt{};
+    opt.useGraph = 1;
+    opt.useTT = useTT;
+    const size_t CH = 1 << 20;
+    for (size_t b = 0; b < ps.size(); b += CH) {
+        int n = (int)std::min(CH, ps.size() - b);
+        PG_TRY(perftBatch(n, ps.data() + b, ds.data() + b, &opt, counts.data() + b));
+    }
+    unsigned __int128 total = 0;
+    for (size_t i = 0; i < ps.size(); i++) total += (unsigned __int128)mult[i] * counts[i];
+    out->lo = (u64)total;
+    out->hi = (u64)(total >> 64);
+    return PG_OK;
+}
+
 static int recurse(const pg_position *pos, int depth, bool useTT, u128 *out)
 {
     Hash128 h{0, 0};
@@ -321,7 +404,10 @@ static int recurse(const pg_position *pos, int depth, bool useTT, u128 *out)
             total = u128(c);
         }
     }
-    if (split) {
+    if (split && g_frontierBatch && depth > 1 && depth - g_effectiveLD <= FRONTIER_MAX_PLIES) {
+        int ld = std::min(g_effectiveLD, depth - 1);
+        PG_TRY(frontierPerft(pos, depth, ld < 1 ? 1 : ld, useTT, &total));
+    } else if (split) {
         if (depth <= 1) {
             // a GPU call of depth 1 cannot run out of memory; depth 0 is a leaf
             total = u128(depth == 0 ? 1 : countMoves(toQBB(pos), pos->state, pos->color));
@@ -456,6 +542,7 @@ int pg_init(int device, uint64_t arenaBytes) { return initLib(device, arenaBytes
 int pg_shutdown(void) { return shutdownLib(); }
 uint64_t pg_arena_bytes(void) { return lib().arenaBytes; }
 void pg_set_arena_limit(uint64_t bytes) { g_arenaLimit = bytes; }
+void pg_set_frontier_batch(int on) { g_frontierBatch = on != 0; }
 
 int pg_parse_fen(const char *fen, pg_position *pos) { return parseFen(fen, pos); }
 int pg_write_fen(const pg_position *pos, char *out, int outLen)

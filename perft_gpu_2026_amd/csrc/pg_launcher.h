@@ -39,5 +39,6 @@ size_t lastBfsPeak();
 // arena bytes a GPU call may use (the arena, or the test limit if set)
 size_t usableArena();
 extern int g_leafPlies;
+int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_batch_options *opt, uint64_t *counts);
 
 }  // namespace pg

@@ -48,11 +48,19 @@ def test_config2_full_depth(gpu):
     assert [gpu.perft_bfs(p, d) for d in range(1, 8)] == r["counts"]
 
 
-def test_launcher_host_recursion(gpu):
+@pytest.mark.parametrize("frontier", [1, 0])
+def test_launcher_host_recursion(gpu, frontier):
+    # frontier=1: host frontier merged and batched; frontier=0: the reference's
+    # serial host recursion with one GPU call per node
     r = load_jsonl("c2.jsonl")[0]
     p = gpu.parse_fen(r["fen"])
-    for ld in (2, 3, 4):
-        assert gpu.perft(p, 6, launch_depth=ld) == r["counts"][5]
+    gpu.lib().pg_set_frontier_batch(frontier)
+    try:
+        for ld in (2, 3, 4):
+            assert gpu.perft(p, 6, launch_depth=ld) == r["counts"][5]
+        assert gpu.perft(gpu.parse_fen(gpu.START_FEN), 7, launch_depth=3) == 3195901860
+    finally:
+        gpu.lib().pg_set_frontier_batch(1)
 
 
 def test_forced_oom_reports_and_falls_back(gpu):
