@@ -122,6 +122,7 @@ static int serveCreate(const pg_serve_config *cfg, void **handle)
     }
     Server *S = new Server();
     S->cfg = *cfg;
+    L.prof.on = false;  // the event profiler is single-threaded; slots run concurrently
     int slots = cfg->maxInflight > 0 ? cfg->maxInflight : 4;
     if (slots > 16) slots = 16;
     u64 budget = cfg->hbmBudgetBytes ? cfg->hbmBudgetBytes : L.arenaBytes;
