@@ -184,6 +184,12 @@ int pg_profile_enable(int on);
 int pg_profile_reset(void);
 int pg_profile_get(pg_kernel_stat *out, int maxOut, int *nOut);
 int pg_device_synchronize(void);
+/* device memory for callers that have no HIP runtime of their own (a second
+ * runtime in the same process cannot share the device): kind 0 = host to
+ * device, 1 = device to host, 2 = device to device; synchronous */
+int pg_device_malloc(uint64_t bytes, void **ptr);
+int pg_device_free(void *ptr);
+int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
 
 /* ============================ batched perft ================================= */
 /* No reference counterpart (the reference perfts one position per call).

@@ -125,6 +125,9 @@ _SIGS = [
     ("pg_profile_reset", c_int, []),
     ("pg_profile_get", c_int, [POINTER(KernelStat), c_int, POINTER(c_int)]),
     ("pg_device_synchronize", c_int, []),
+    ("pg_device_malloc", c_int, [c_uint64, POINTER(c_void_p)]),
+    ("pg_device_free", c_int, [c_void_p]),
+    ("pg_memcpy", c_int, [c_void_p, c_void_p, c_uint64, c_int]),
 ]
 
 
@@ -323,3 +326,45 @@ def profile_get() -> dict[str, dict]:
 
 def device_synchronize() -> None:
     _check(lib().pg_device_synchronize(), "pg_device_synchronize")
+
+
+class DeviceArray:
+    """A device buffer owned through the library (numpy in, numpy out)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = max(int(nbytes), 1)
+        self.ptr = c_void_p()
+        _check(lib().pg_device_malloc(self.nbytes, byref(self.ptr)), "pg_device_malloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes:
+            _check(lib().pg_memcpy(d.ptr, a.ctypes.data_as(c_void_p), a.nbytes, 0), "pg_memcpy")
+        return d
+
+    def to_numpy(self, dtype, count):
+        import numpy as np
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(lib().pg_memcpy(out.ctypes.data_as(c_void_p), self.ptr, out.nbytes, 1), "pg_memcpy")
+        return out
+
+    def zero(self):
+        import numpy as np
+        z = np.zeros(self.nbytes, dtype=np.uint8)
+        _check(lib().pg_memcpy(self.ptr, z.ctypes.data_as(c_void_p), self.nbytes, 0), "pg_memcpy")
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib().pg_device_free(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass

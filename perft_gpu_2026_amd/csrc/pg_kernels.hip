@@ -916,3 +916,29 @@ extern "C" int pg_device_synchronize(void)
     PG_HIP(hipDeviceSynchronize());
     return PG_OK;
 }
+
+// device memory helpers for callers without a HIP runtime of their own
+// (the kernel-level entry points take device pointers)
+extern "C" int pg_device_malloc(uint64_t bytes, void **ptr)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if (!ptr) return PG_ERR_INVALID;
+    PG_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return PG_OK;
+}
+
+extern "C" int pg_device_free(void *ptr)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    PG_HIP(hipFree(ptr));
+    return PG_OK;
+}
+
+extern "C" int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind)
+{
+    if (!lib().init) return PG_ERR_NOT_INIT;
+    if (kind < 0 || kind > 2 || (bytes && (!dst || !src))) return PG_ERR_INVALID;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    PG_HIP(hipMemcpy(dst, src, bytes, k));
+    return PG_OK;
+}

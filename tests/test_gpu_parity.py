@@ -65,10 +65,12 @@ def test_launcher_host_recursion(gpu, frontier):
 
 def test_forced_oom_reports_and_falls_back(gpu):
     kiwi = gpu.parse_fen("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -")
-    import torch
-    buf = torch.empty(64 * 1024, dtype=torch.uint8, device="cuda")
-    s, _ = gpu.perft_bfs_status(kiwi, 5, buf.data_ptr(), buf.numel())
+    buf = gpu.DeviceArray(64 * 1024)
+    s, _ = gpu.perft_bfs_status(kiwi, 5, buf.ptr.value, buf.nbytes)
     assert s == gpu.PG_ERR_ARENA_OOM
+    # a buffer that fits gives the count
+    big = gpu.DeviceArray(256 << 20)
+    assert gpu.perft_bfs_status(kiwi, 4, big.ptr.value, big.nbytes) == (0, 4085603)
     before = gpu.lib().pg_get_oom_fallback_count()
     gpu.lib().pg_set_arena_limit(8 << 20)
     try:
@@ -86,58 +88,44 @@ def test_config3_sample_plain_batch(gpu):
 
 
 # ---------------------------------------------------------------- kernel ABI
-def _stream():
-    import torch
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-
-
-def _p(t):
-    return ctypes.c_void_p(t.data_ptr())
+# Device buffers come from the library itself (pg_device_malloc/pg_memcpy):
+# a second HIP runtime in this process (torch's) cannot share the device.
+def _d(gpu, a):
+    return gpu.DeviceArray.from_numpy(a)
 
 
 @pytest.mark.parametrize("n", [1, 255, 256, 257, 100_000, 3_000_001])
 def test_scan_matches_cumsum(gpu, n):
-    import torch
     rng = np.random.default_rng(n)
     host = rng.integers(0, 219, size=n, dtype=np.uint32)
-    d = torch.from_numpy(host.astype(np.int32)).cuda()
+    d = _d(gpu, host)
     tiles = (n + 255) // 256
-    scratch = torch.zeros(8 * (tiles + 1) + 4 * tiles + 256, dtype=torch.uint8, device="cuda")
+    scratch = gpu.DeviceArray(8 * (tiles + 1) + 4 * tiles + 256)
     total = ctypes.c_uint64()
-    assert gpu.lib().pg_scan_u32(_p(d), n, None, _p(scratch), ctypes.byref(total), _stream()) == 0
-    torch.cuda.synchronize()
-    assert np.array_equal(d.cpu().numpy().view(np.uint32), np.cumsum(host, dtype=np.uint64).astype(np.uint32))
+    assert gpu.lib().pg_scan_u32(d.ptr, n, None, scratch.ptr, ctypes.byref(total), None) == 0
+    gpu.device_synchronize()
+    assert np.array_equal(d.to_numpy(np.uint32, n), np.cumsum(host, dtype=np.uint64).astype(np.uint32))
     assert total.value == int(host.sum(dtype=np.uint64))
 
 
 @pytest.mark.parametrize("n,zero_frac", [(1, 0.0), (1000, 0.0), (50_000, 0.3), (20_000, 0.97)])
 def test_expand_matches_searchsorted(gpu, n, zero_frac):
-    import torch
     rng = np.random.default_rng(n)
     counts = rng.integers(1, 60, size=n).astype(np.uint32)
     counts[rng.random(n) < zero_frac] = 0
     counts[-1] = max(counts[-1], 1)
     incl = np.cumsum(counts, dtype=np.uint64).astype(np.uint32)
     c = int(incl[-1])
-    d_incl = torch.from_numpy(incl.view(np.int32)).cuda()
-    out = torch.empty(c, dtype=torch.int32, device="cuda")
-    scratch = torch.empty(4 * (c // 2048 + 2), dtype=torch.uint8, device="cuda")
-    assert gpu.lib().pg_expand(_p(d_incl), n, c, _p(out), _p(scratch), _stream()) == 0
-    torch.cuda.synchronize()
+    d_incl = _d(gpu, incl)
+    out = gpu.DeviceArray(4 * c)
+    scratch = gpu.DeviceArray(4 * (c // 2048 + 2))
+    assert gpu.lib().pg_expand(d_incl.ptr, n, c, out.ptr, scratch.ptr, None) == 0
+    gpu.device_synchronize()
     want = np.searchsorted(incl, np.arange(c), side="right")
-    assert np.array_equal(out.cpu().numpy(), want)
+    assert np.array_equal(out.to_numpy(np.uint32, c), want)
 
 
-def _level_inputs(gpu, orc, nroots):
-    roots = [gpu.random_position(s, 5 + s % 50) for s in range(nroots)]
-    boards = np.array([list(p.bb) for p in roots], dtype=np.uint64)
-    states = np.array([p.state for p in roots], dtype=np.uint8)
-    colors = [p.color for p in roots]
-    return roots, boards, states, colors
-
-
-def test_makecount_and_genmoves_match_host(gpu, orc):
-    import torch
+def test_makecount_and_genmoves_match_host(gpu):
     # one colour per call: white-to-move parents
     roots = [p for p in (gpu.random_position(s, 2 * (s % 30)) for s in range(400)) if p.color == 0]
     pidx, moves, children = [], [], []
@@ -147,31 +135,28 @@ def test_makecount_and_genmoves_match_host(gpu, orc):
             moves.append(m)
             children.append(gpu.make_move(p, m))
     n = len(moves)
-    pb = torch.from_numpy(np.array([list(p.bb) for p in roots], dtype=np.uint64).view(np.int64)).cuda()
-    ps = torch.from_numpy(np.array([p.state for p in roots], dtype=np.uint8)).cuda()
-    di = torch.from_numpy(np.array(pidx, dtype=np.int32)).cuda()
-    dm = torch.from_numpy(np.array(moves, dtype=np.uint16).view(np.int16)).cuda()
-    ob = torch.empty(n * 4, dtype=torch.int64, device="cuda")
-    os_ = torch.empty(n, dtype=torch.uint8, device="cuda")
-    cnt = torch.empty(n, dtype=torch.int32, device="cuda")
-    tiles = torch.empty((n + 255) // 256, dtype=torch.int32, device="cuda")
     L = gpu.lib()
-    assert L.pg_level_makecount(_p(pb), _p(ps), _p(di), _p(dm), 0, n, _p(ob), _p(os_), _p(cnt), _p(tiles),
-                                _stream()) == 0
-    torch.cuda.synchronize()
-    got_b = ob.cpu().numpy().view(np.uint64).reshape(n, 4)
-    got_c = cnt.cpu().numpy()
+    pb = _d(gpu, np.array([list(p.bb) for p in roots], dtype=np.uint64))
+    ps = _d(gpu, np.array([p.state for p in roots], dtype=np.uint8))
+    di = _d(gpu, np.array(pidx, dtype=np.uint32))
+    dm = _d(gpu, np.array(moves, dtype=np.uint16))
+    ob, os_ = gpu.DeviceArray(32 * n), gpu.DeviceArray(n)
+    cnt, tiles = gpu.DeviceArray(4 * n), gpu.DeviceArray(4 * ((n + 255) // 256))
+    assert L.pg_level_makecount(pb.ptr, ps.ptr, di.ptr, dm.ptr, 0, n, ob.ptr, os_.ptr, cnt.ptr, tiles.ptr, None) == 0
+    gpu.device_synchronize()
+    got_b = ob.to_numpy(np.uint64, 4 * n).reshape(n, 4)
+    got_c = cnt.to_numpy(np.uint32, n)
     for k, ch in enumerate(children):
         assert tuple(int(x) for x in got_b[k]) == tuple(ch.bb)
         assert got_c[k] == gpu.count_moves(ch)
-    assert int(tiles.cpu().numpy().astype(np.int64).sum()) == int(got_c.sum())
+    assert int(tiles.to_numpy(np.uint32, (n + 255) // 256).astype(np.int64).sum()) == int(got_c.sum())
     # genmoves over the children (black to move)
-    incl = torch.from_numpy(np.cumsum(got_c).astype(np.int32)).cuda()
+    incl = _d(gpu, np.cumsum(got_c).astype(np.uint32))
     c = int(got_c.sum())
-    out = torch.empty(c, dtype=torch.int16, device="cuda")
-    assert L.pg_genmoves(_p(ob), _p(os_), _p(incl), 1, n, _p(out), _stream()) == 0
-    torch.cuda.synchronize()
-    got_m = out.cpu().numpy().view(np.uint16)
+    out = gpu.DeviceArray(2 * c)
+    assert L.pg_genmoves(ob.ptr, os_.ptr, incl.ptr, 1, n, out.ptr, None) == 0
+    gpu.device_synchronize()
+    got_m = out.to_numpy(np.uint16, c)
     off = 0
     for k, ch in enumerate(children):
         seg = sorted(int(x) for x in got_m[off:off + got_c[k]])
@@ -180,18 +165,17 @@ def test_makecount_and_genmoves_match_host(gpu, orc):
 
 
 def test_leaf_entry_point(gpu):
-    import torch
     p = gpu.parse_fen("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -")
     mv = gpu.gen_moves(p)
-    pb = torch.from_numpy(np.array([list(p.bb)], dtype=np.uint64).view(np.int64)).cuda()
-    ps = torch.tensor([p.state], dtype=torch.uint8, device="cuda")
-    di = torch.zeros(len(mv), dtype=torch.int32, device="cuda")
-    dm = torch.from_numpy(np.array(mv, dtype=np.uint16).view(np.int16)).cuda()
+    pb = _d(gpu, np.array([list(p.bb)], dtype=np.uint64))
+    ps = _d(gpu, np.array([p.state], dtype=np.uint8))
+    di = _d(gpu, np.zeros(len(mv), dtype=np.uint32))
+    dm = _d(gpu, np.array(mv, dtype=np.uint16))
     for plies, want in ((1, 2039), (2, 97862)):
-        s = torch.zeros(1, dtype=torch.int64, device="cuda")
-        assert gpu.lib().pg_leaf(_p(pb), _p(ps), _p(di), _p(dm), 0, len(mv), plies, _p(s), _stream()) == 0
-        torch.cuda.synchronize()
-        assert int(s.item()) == want
+        s = gpu.DeviceArray(8).zero()
+        assert gpu.lib().pg_leaf(pb.ptr, ps.ptr, di.ptr, dm.ptr, 0, len(mv), plies, s.ptr, None) == 0
+        gpu.device_synchronize()
+        assert int(s.to_numpy(np.uint64, 1)[0]) == want
 
 
 def test_cli_gpu(gpu):
@@ -200,3 +184,7 @@ def test_cli_gpu(gpu):
     assert r.returncode == 0, r.stdout + r.stderr
     assert gpu.parse_cli_counts(r.stdout) == {1: 48, 2: 2039, 3: 97862, 4: 4085603, 5: 193690690}
     assert "Launch depth:" in r.stdout
+    r = gpu.run_cli(["rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1", "6", "-arena", "4096",
+                     "-dtt", "512", "-htt", "256"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert gpu.parse_cli_counts(r.stdout)[6] == 119060324
