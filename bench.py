@@ -167,6 +167,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--leaf", type=int, default=0)
+    ap.add_argument("--leaf2-variant", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--serve", type=int, default=0)
@@ -186,6 +187,7 @@ def main():
     pg.init(0)
     if a.leaf:
         pg.set_leaf_plies(a.leaf)
+    pg.lib().pg_set_leaf2_variant(a.leaf2_variant)
     pos = pg.parse_fen(fx["fen"])
     for i in range(a.warmup):
         c = pg.perft_bfs(pos, depth)

@@ -201,9 +201,9 @@ constexpr int L2_CAP = 1536;
 constexpr int L2_WAVES = 4;
 
 template <int C>
-__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__restrict__ pb,
-                                                         const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
-                                                         const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum)
+__device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
+                                          const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
+                                          u64 *__restrict__ sum)
 {
     __shared__ QuadBitBoard sb[L2_WAVES][L2_CHUNK];
     __shared__ u8 ss[L2_WAVES][L2_CHUNK];
@@ -254,6 +254,27 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__r
     u64 s = waveSum64(acc);
     if (lane == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
 }
+
+// Two register budgets for the same leaf: the compiler's own choice (about
+// 156 VGPRs, 3 waves/SIMD) and one capped for 4 waves/SIMD (128 VGPRs, a small
+// spill in the generation phase).  pg_set_leaf2_variant picks; the autotuner
+// and bench measure both.
+template <int C>
+__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__restrict__ pb,
+                                                         const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                         const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum)
+{
+    leaf2Body<C>(pb, ps, pidx, mv, n, sum);
+}
+template <int C>
+__global__ __launch_bounds__(L2_WAVES * 64, 4) void k_leaf2_w4(const QuadBitBoard *__restrict__ pb,
+                                                               const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                               const u16 *__restrict__ mv, u64 n,
+                                                               u64 *__restrict__ sum)
+{
+    leaf2Body<C>(pb, ps, pidx, mv, n, sum);
+}
+int g_leaf2Variant = 0;
 
 // ------------------------------------------------------------ TT-mode kernels
 // With a transposition table each expanded level probes the table at its
@@ -541,8 +562,13 @@ static int launchLeaf(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, con
         u64 blocks = (chunks + L2_WAVES - 1) / L2_WAVES;
         u64 cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 8;
         dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(L2_WAVES * 64);
-        if (color == WHITE) hipLaunchKernelGGL(k_leaf2<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
-        else hipLaunchKernelGGL(k_leaf2<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        if (g_leaf2Variant == 1) {
+            if (color == WHITE) hipLaunchKernelGGL(k_leaf2_w4<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+            else hipLaunchKernelGGL(k_leaf2_w4<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        } else {
+            if (color == WHITE) hipLaunchKernelGGL(k_leaf2<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+            else hipLaunchKernelGGL(k_leaf2<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        }
     }
     PG_HIP(hipGetLastError());
     return PG_OK;
@@ -942,3 +968,5 @@ extern "C" int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind)
     PG_HIP(hipMemcpy(dst, src, bytes, k));
     return PG_OK;
 }
+
+extern "C" void pg_set_leaf2_variant(int v) { g_leaf2Variant = v == 1 ? 1 : 0; }
