@@ -302,17 +302,22 @@ __global__ __launch_bounds__(BS) void k_b_genmoves(BatchDev *d, int L)
         u64 blockStart = b0 ? incl[b0 - 1] : 0;
         u64 total = (u64)incl[bEnd - 1] - blockStart;
         u64 excl = (i < n && i) ? (u64)incl[i - 1] : 0;
+        u64 mine = i < n ? (u64)incl[i] - excl : 0;  // 0 for a table hit: nothing to write
         if (total <= GEN_LCAP) {
-            if (i < n) {
+            if (mine) {
                 u16 *dst = buf + (excl - blockStart);
-                generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) { dst[k] = m; });
+                generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) {
+                    if (k < mine) dst[k] = m;
+                });
             }
             __syncthreads();
             u16 *o = out + blockStart;
             for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
-        } else if (i < n) {
+        } else if (mine) {
             u16 *dst = out + excl;
-            generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) { dst[k] = m; });
+            generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) {
+                if (k < mine) dst[k] = m;
+            });
         }
         __syncthreads();
     }

@@ -150,19 +150,26 @@ __global__ __launch_bounds__(BS) void k_genmoves(const QuadBitBoard *__restrict_
     u64 bEnd = b0 + BS < n ? b0 + BS : n;
     u64 blockStart = b0 ? incl[b0 - 1] : 0;
     u64 total = (u64)incl[bEnd - 1] - blockStart;
-    u64 excl = 0;
-    if (i < n) excl = (i ? (u64)incl[i - 1] : 0);
+    u64 excl = 0, mine = 0;
+    if (i < n) {
+        excl = i ? (u64)incl[i - 1] : 0;
+        mine = (u64)incl[i] - excl;  // 0 for a table hit or a duplicate: nothing to write
+    }
     if (total <= GEN_LCAP) {
-        if (i < n) {
+        if (mine) {
             u16 *dst = buf + (excl - blockStart);
-            generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) { dst[k] = m; });
+            generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) {
+                if (k < mine) dst[k] = m;
+            });
         }
         __syncthreads();
         u16 *o = out + blockStart;
         for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
-    } else if (i < n) {
+    } else if (mine) {
         u16 *dst = out + excl;
-        generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) { dst[k] = m; });
+        generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) {
+            if (k < mine) dst[k] = m;
+        });
     }
 }
 

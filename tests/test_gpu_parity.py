@@ -188,3 +188,27 @@ def test_cli_gpu(gpu):
                      "-dtt", "512", "-htt", "256"])
     assert r.returncode == 0, r.stdout + r.stderr
     assert gpu.parse_cli_counts(r.stdout)[6] == 119060324
+
+
+def test_genmoves_writes_nothing_for_zero_count_entries(gpu):
+    # entries whose count is 0 (a table hit or a duplicate in TT mode) still
+    # have legal moves; genmoves must not write them over their neighbours'
+    ps = [gpu.random_position(s, 10 + s % 20) for s in range(600)]
+    ps = [p for p in ps if p.color == 0]
+    n = len(ps)
+    counts = np.array([gpu.count_moves(p) for p in ps], dtype=np.uint32)
+    counts[::3] = 0
+    incl = np.cumsum(counts).astype(np.uint32)
+    c = int(incl[-1])
+    b = _d(gpu, np.array([list(p.bb) for p in ps], dtype=np.uint64))
+    s = _d(gpu, np.array([p.state for p in ps], dtype=np.uint8))
+    d_incl = _d(gpu, incl)
+    out = _d(gpu, np.full(c + 64, 0xFFFF, dtype=np.uint16))
+    assert gpu.lib().pg_genmoves(b.ptr, s.ptr, d_incl.ptr, 0, n, out.ptr, None) == 0
+    gpu.device_synchronize()
+    got = out.to_numpy(np.uint16, c + 64)
+    off = 0
+    for k, p in enumerate(ps):
+        assert sorted(int(x) for x in got[off:off + counts[k]]) == (sorted(gpu.gen_moves(p)) if counts[k] else [])
+        off += counts[k]
+    assert (got[c:] == 0xFFFF).all()
