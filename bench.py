@@ -226,7 +226,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
-                   "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 1},
+                   "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
         "roofline": roofline(leaf) if leaf else None,
         "roofline_levels": roofline(top_level) if top_level else None,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),

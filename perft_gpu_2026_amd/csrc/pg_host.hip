@@ -39,7 +39,7 @@ static int g_effectiveLD = 0;
 static std::atomic<uint64_t> g_oomCount{0};
 static size_t g_lastPeak = 0;
 static uint64_t g_arenaLimit = 0;  // 0 = whole arena; a test hook to force OOM
-int g_leafPlies = 1;
+int g_leafPlies = 2;  // fused two-ply leaf (measured 2x the one-ply leaf on config 2)
 static bool g_frontierBatch = true;  // launcher: batch the frontier above the launch depth
 static const int FRONTIER_MAX_PLIES = 4;
 

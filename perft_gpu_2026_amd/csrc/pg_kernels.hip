@@ -174,22 +174,23 @@ __global__ __launch_bounds__(BS) void k_genmoves(const QuadBitBoard *__restrict_
 }
 
 // One-ply leaf (makeMove_and_perft_leaf_kernel, perft_kernels.cu:313-350):
-// perft(1) of make(parent, move) per entry.
+// perft(1) of make(parent, move) per entry.  Persistent grid-stride loop with
+// per-thread sums and one u64 atomic per block at the end (one atomic per
+// 256 entries serialised on a single address cost more than the counting).
 template <int C>
 __global__ __launch_bounds__(BS) void k_leaf1(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
                                               const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
                                               u64 *__restrict__ sum)
 {
-    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
-    u32 v = 0;
-    if (i < n) {
+    u64 acc = 0;
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < n; i += (u64)gridDim.x * BS) {
         u32 p = pidx[i];
         QuadBitBoard q = pb[p];
         u8 st = ps[p];
         makeMove<C>(q, st, mv[i]);
-        v = countMoves<1 - C>(q, st);
+        acc += countMoves<1 - C>(q, st);
     }
-    u64 s = blockSum64(v);
+    u64 s = blockSum64(acc);
     if (threadIdx.x == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
 }
 
@@ -560,7 +561,8 @@ static int launchLeaf(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, con
 {
     if (!n) return PG_OK;
     if (plies == 1) {
-        dim3 g(gridFor(n, BS)), b(BS);
+        u64 blocks = (n + BS - 1) / BS, cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 16;
+        dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(BS);
         if (color == WHITE) hipLaunchKernelGGL(k_leaf1<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
         else hipLaunchKernelGGL(k_leaf1<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
     } else {

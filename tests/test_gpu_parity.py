@@ -32,7 +32,7 @@ def test_known_positions_bfs(gpu, leaf):
             print(f"[gpu] bfs leaf={leaf} {name} d={d}", flush=True)
             assert gpu.perft_bfs(gpu.parse_fen(fen), d) == c, (name, d)
     finally:
-        gpu.set_leaf_plies(1)
+        gpu.set_leaf_plies(2)
 
 
 def test_config1_all_depths(gpu):

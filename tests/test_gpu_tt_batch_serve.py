@@ -33,7 +33,7 @@ def test_tt_bfs_equals_no_tt(tt, leaf):
         st = tt.tt_stats()
         assert st[0] > 0 and st[1] > 0 and st[2] > 0, st
     finally:
-        tt.set_leaf_plies(1)
+        tt.set_leaf_plies(2)
 
 
 def test_tt_launcher_with_host_tier(tt):
