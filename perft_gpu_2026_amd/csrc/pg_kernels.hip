@@ -661,6 +661,11 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         PG_HIP(hipStreamSynchronize(st));
         u64 c = pinned[0];
         lv.push_back({ob, os, cnt, known, redir, val, n});
+        if (c == 0) {
+            // every entry was a table hit, a duplicate or terminal: nothing below
+            n = 0;
+            break;
+        }
         if (c >= 0xFFFFFFFFull) return PG_ERR_ARENA_OOM;
         u32 *parts = A.alloc<u32>((c + EXPAND_TILE - 1) / EXPAND_TILE + 1);
         u32 *nidx = A.alloc<u32>(c);
@@ -676,11 +681,11 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         mv = nmv;
         n = c;
     }
-    // leaf values
+    // leaf values (none when a level above came out empty)
     int pc = color ^ ((last - 1) & 1);
     u64 *leafVal = A.alloc<u64>(n);
     if (!leafVal) return PG_ERR_ARENA_OOM;
-    if (n) {
+    if (n && (int)lv.size() == last - 1) {
         if (leafPlies == 1) {
             dim3 g(gridFor(n, BS)), b(BS);
             if (pc == WHITE) hipLaunchKernelGGL(k_leaf1_val<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, leafVal);
@@ -695,9 +700,9 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         }
         PG_HIP(hipGetLastError());
     }
-    // upsweep: level `last-1` down to 1
+    // upsweep: the deepest built level down to 1
     const u64 *childVal = leafVal;
-    for (int lvl = last - 1; lvl >= 1; lvl--) {
+    for (int lvl = (int)lv.size(); lvl >= 1; lvl--) {
         TTLevel &l = lv[lvl - 1];
         int side = color ^ (lvl & 1);
         dim3 g(gridFor(l.n, BS)), b(BS);
