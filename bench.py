@@ -64,12 +64,20 @@ def cpu_baseline(fen: str, depth: int, budget_s: float = 15.0) -> dict:
                       f"config-2 position: {nodes} nodes in {dt:.1f} s, single-threaded like the reference -cpu path"}
 
 
-def roofline(stat: dict) -> dict:
+# HBM bytes per launch of the default config-2 leaf, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of this bench, each scaled by
+# what the counter reads for a plain 256 MiB-in / 256 MiB-out elementwise
+# kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0).
+# Commands and figures: profiles/leaf2_config2_pmc.md.
+PMC_TRAFFIC = {"leaf2": 61639.1875 * 1024 * 2.0 + 256.0 * 1024 * 1.0}
+
+
+def roofline(stat: dict, traffic=None) -> dict:
     avg_ms = stat["ms"] / stat["launches"]
     bytes_per_launch = stat["bytes"] / stat["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": stat["name"], "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_bytes_per_launch": round(bytes_per_launch)}
 
@@ -227,7 +235,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
                    "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
-        "roofline": roofline(leaf) if leaf else None,
+        "roofline": roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None) if leaf else None,
         "roofline_levels": roofline(top_level) if top_level else None,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
