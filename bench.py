@@ -130,6 +130,8 @@ def serve_leg(pg, n_req: int, rate: float) -> dict:
                 for i in range(n.value):
                     r = res[i]
                     done[r.id] = (r.count, r.submitTime, r.startTime, r.endTime, r.status)
+                    if len(done) % 50 == 0:
+                        print(f"[bench] serving: {len(done)}/{len(reqs)} done", file=sys.stderr, flush=True)
                 if stop.is_set() and len(done) >= len(reqs):
                     break
 
@@ -151,6 +153,7 @@ def serve_leg(pg, n_req: int, rate: float) -> dict:
     first = min(v[1] for v in done.values())
     last = max(v[3] for v in done.values())
     capacity = len(reqs) / (last - first)
+    print(f"[bench] serving closed loop: {len(reqs)} requests in {last - first:.2f} s", file=sys.stderr, flush=True)
     lam = rate or 0.75 * capacity
     rnd = random.Random(5)
     t, arrivals = 0.0, []

@@ -122,8 +122,8 @@ uint64_t pg_last_bfs_peak_bytes(void);
 /* leaf plies of the GPU BFS: 2 = fused two-ply leaf (default), 1 = one-ply leaf
  * over a materialised last level (switches.h:21 USE_FUSED_2LEVEL_LEAF) */
 void pg_set_leaf_plies(int plies);
-/* register budget of the fused two-ply leaf: 0 = compiler's choice, 1 = capped
- * for 4 waves per SIMD */
+/* shape of the fused two-ply leaf: 0 = 64 entries per wave (default),
+ * 1 = 32 entries per wave */
 void pg_set_leaf2_variant(int variant);
 /* decimal text of a 128-bit count (uint128.h:46-108) */
 int pg_u128_to_string(uint64_t lo, uint64_t hi, char *buf, int n);

@@ -36,7 +36,7 @@
 namespace pg {
 
 constexpr int MAXLV = 24;
-constexpr int B_CHUNK = 32;  // leaf entries per work-queue pop
+constexpr int B_CHUNK = 64;  // leaf entries per work-queue pop (one per lane)
 
 struct LvlDev {
     u64 n;                 // entries (level 0: roots)
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
 // persistent fused two-ply leaf: the wave-cooperative scheme of k_leaf2
 // (child boards and the flat pair list in LDS) with work-queue chunks and
 // per-entry partial sums kept in LDS until the chunk is done
-constexpr int BL2_CAP = 1536;
+constexpr int BL2_CAP = 2048;
 constexpr int BL2_WAVES = 4;
 template <int C>
 __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)

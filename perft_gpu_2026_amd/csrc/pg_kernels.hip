@@ -203,26 +203,30 @@ __global__ __launch_bounds__(BS) void k_leaf1(const QuadBitBoard *__restrict__ p
 // per-thread move list is kept (the reference's CMove[80] stack array and its
 // 80-move cap are gone), and generation and counting live in separate loops
 // so the counter's register budget is not shared with the generator's.
-// Pair lists longer than L2_CAP are processed in windows.
-constexpr int L2_CHUNK = 32;
-constexpr int L2_CAP = 1536;
+// Pair lists longer than the LDS capacity are processed in windows.  A chunk
+// of 64 entries keeps every lane busy in the child phase (make, count,
+// generate); 64 entries have about 1,750 pairs, so 2,048 pair slots rarely
+// need a second window.  41 KB of LDS per 4-wave block, 3 blocks per CU.
+constexpr int L2_CHUNK = 64;
+constexpr int L2_CAP = 2048;
 constexpr int L2_WAVES = 4;
 
-template <int C>
+template <int C, int CHUNK, int CAP>
 __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
                                           const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
                                           u64 *__restrict__ sum)
 {
-    __shared__ QuadBitBoard sb[L2_WAVES][L2_CHUNK];
-    __shared__ u8 ss[L2_WAVES][L2_CHUNK];
-    __shared__ u32 sl[L2_WAVES][L2_CAP];
+    static_assert(CHUNK <= 64, "one entry per lane");
+    __shared__ QuadBitBoard sb[L2_WAVES][CHUNK];
+    __shared__ u8 ss[L2_WAVES][CHUNK];
+    __shared__ u32 sl[L2_WAVES][CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     u64 acc = 0;
-    const u64 stride = (u64)gridDim.x * L2_WAVES * L2_CHUNK;
-    for (u64 base = ((u64)blockIdx.x * L2_WAVES + w) * L2_CHUNK; base < n; base += stride) {
+    const u64 stride = (u64)gridDim.x * L2_WAVES * CHUNK;
+    for (u64 base = ((u64)blockIdx.x * L2_WAVES + w) * CHUNK; base < n; base += stride) {
         u32 cnt = 0;
         u64 e = base + lane;
-        if (lane < L2_CHUNK && e < n) {
+        if (lane < CHUNK && e < n) {
             u32 p = pidx[e];
             QuadBitBoard q = pb[p];
             u8 st = ps[p];
@@ -235,19 +239,19 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
         u32 off = incl - cnt;
         u32 total = __shfl(incl, 63);
         waveLdsSync();
-        for (u32 wb = 0; wb < total; wb += L2_CAP) {
-            if (cnt && off < wb + L2_CAP && off + cnt > wb) {
+        for (u32 wb = 0; wb < total; wb += CAP) {
+            if (cnt && off < wb + CAP && off + cnt > wb) {
                 QuadBitBoard q = sb[w][lane];
                 u8 st = ss[w][lane];
                 u32 *dst = sl[w];
                 const u32 tag = (u32)lane << 16;
                 generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;  // wraps (and is skipped) below the window
-                    if (j < L2_CAP) dst[j] = (u32)m | tag;
+                    if (j < (u32)CAP) dst[j] = (u32)m | tag;
                 });
             }
             waveLdsSync();
-            u32 wn = total - wb < (u32)L2_CAP ? total - wb : (u32)L2_CAP;
+            u32 wn = total - wb < (u32)CAP ? total - wb : (u32)CAP;
             for (u32 j = lane; j < wn; j += 64) {
                 u32 x = sl[w][j];
                 u32 src = x >> 16;
@@ -263,24 +267,23 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
     if (lane == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
 }
 
-// Two register budgets for the same leaf: the compiler's own choice (about
-// 156 VGPRs, 3 waves/SIMD) and one capped for 4 waves/SIMD (128 VGPRs, a small
-// spill in the generation phase).  pg_set_leaf2_variant picks; the autotuner
-// and bench measure both.
+// Two shapes of the same leaf: 64-entry chunks (default) and 32-entry chunks
+// with a 1,536-pair list (half the child phase's lanes idle; measured 8 %
+// slower on the config-2 tree).  pg_set_leaf2_variant picks.
 template <int C>
 __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__restrict__ pb,
                                                          const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
                                                          const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum)
 {
-    leaf2Body<C>(pb, ps, pidx, mv, n, sum);
+    leaf2Body<C, L2_CHUNK, L2_CAP>(pb, ps, pidx, mv, n, sum);
 }
 template <int C>
-__global__ __launch_bounds__(L2_WAVES * 64, 4) void k_leaf2_w4(const QuadBitBoard *__restrict__ pb,
-                                                               const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
-                                                               const u16 *__restrict__ mv, u64 n,
-                                                               u64 *__restrict__ sum)
+__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_c32(const QuadBitBoard *__restrict__ pb,
+                                                             const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                             const u16 *__restrict__ mv, u64 n,
+                                                             u64 *__restrict__ sum)
 {
-    leaf2Body<C>(pb, ps, pidx, mv, n, sum);
+    leaf2Body<C, 32, 1536>(pb, ps, pidx, mv, n, sum);
 }
 int g_leaf2Variant = 0;
 
@@ -567,13 +570,14 @@ static int launchLeaf(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, con
         else hipLaunchKernelGGL(k_leaf1<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
     } else {
         // persistent-style grid: enough waves to fill every CU, each wave strides over chunks
-        u64 chunks = (n + L2_CHUNK - 1) / L2_CHUNK;
+        const u64 chunk = g_leaf2Variant == 1 ? 32 : L2_CHUNK;
+        u64 chunks = (n + chunk - 1) / chunk;
         u64 blocks = (chunks + L2_WAVES - 1) / L2_WAVES;
         u64 cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 8;
         dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(L2_WAVES * 64);
         if (g_leaf2Variant == 1) {
-            if (color == WHITE) hipLaunchKernelGGL(k_leaf2_w4<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
-            else hipLaunchKernelGGL(k_leaf2_w4<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+            if (color == WHITE) hipLaunchKernelGGL(k_leaf2_c32<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+            else hipLaunchKernelGGL(k_leaf2_c32<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
         } else {
             if (color == WHITE) hipLaunchKernelGGL(k_leaf2<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
             else hipLaunchKernelGGL(k_leaf2<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);

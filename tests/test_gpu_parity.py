@@ -35,6 +35,20 @@ def test_known_positions_bfs(gpu, leaf):
         gpu.set_leaf_plies(2)
 
 
+@pytest.mark.parametrize("variant", [1])
+def test_leaf2_variants(gpu, variant):
+    # the 32-entry-chunk shape of the fused leaf (the default takes 64)
+    gpu.lib().pg_set_leaf2_variant(variant)
+    try:
+        for name, fen, d, c in _known(2_000_000_000):
+            print(f"[gpu] leaf2 variant={variant} {name} d={d}", flush=True)
+            assert gpu.perft_bfs(gpu.parse_fen(fen), d) == c, (name, d)
+        r = load_jsonl("c2.jsonl")[0]
+        assert gpu.perft_bfs(gpu.parse_fen(r["fen"]), 7) == r["counts"][6]
+    finally:
+        gpu.lib().pg_set_leaf2_variant(0)
+
+
 def test_config1_all_depths(gpu):
     r = load_jsonl("c1.jsonl")[0]
     p = gpu.parse_fen(r["fen"])
@@ -72,9 +86,11 @@ def test_forced_oom_reports_and_falls_back(gpu):
     big = gpu.DeviceArray(256 << 20)
     assert gpu.perft_bfs_status(kiwi, 4, big.ptr.value, big.nbytes) == (0, 4085603)
     before = gpu.lib().pg_get_oom_fallback_count()
+    # perft 6 with the two-ply leaf holds ~4.1 M leaf entries (25 MB): an 8 MB
+    # arena cannot take it, so the launcher splits one ply down
     gpu.lib().pg_set_arena_limit(8 << 20)
     try:
-        assert gpu.perft(kiwi, 5, launch_depth=5) == 193690690
+        assert gpu.perft(kiwi, 6, launch_depth=6) == 8031647685
     finally:
         gpu.lib().pg_set_arena_limit(0)
     assert gpu.lib().pg_get_oom_fallback_count() > before
