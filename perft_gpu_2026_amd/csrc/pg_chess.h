@@ -13,14 +13,19 @@
 // Design for CDNA4 (own design, not the reference's): every routine is
 // branch-light so a 64-lane wavefront stays converged.
 //   * slider attacks are computed, not looked up: per-direction Kogge-Stone
-//     occluded fills for whole piece sets, and hyperbola quintessence with a
-//     full 64-bit bit reverse (v_bfrev_b32 x2) for single-square lines;
-//     no table reads, so the hot loop is pure VALU.
+//     occluded fills for whole piece sets, the rank-wise subtraction trick for
+//     east/west, and hyperbola quintessence with a full 64-bit bit reverse for
+//     single-square lines; the only table is 64 rows of per-square masks.
+//   * set arithmetic runs on 32-bit halves (BB below): gfx950's
+//     v_bitop3_b32 evaluates any three-input bitwise function in one
+//     instruction, and the compiler forms it only for 32-bit operations;
+//     64-bit shifts stay single v_lshlrev_b64 / v_lshrrev_b64 instructions.
 //   * pins and checks come from the few enemy sliders on the king's lines;
 //     pinned pieces are handled by per-line source masks instead of per-piece
 //     loops (a piece pinned on a line may move only along it).
-//   * legal-move counting is set-wise: knight, slider and pawn counts are
-//     popcounts of per-direction target sets; only en passant (rare) branches.
+//   * legal-move counting is set-wise: slider and pawn counts are popcounts
+//     of per-direction target sets, knights one table row each; only en
+//     passant (rare) branches.
 // The semantics follow the reference's countMoves/makeMove
 // (MoveGeneratorBitboard.h:1563-2067), including its en-passant rules
 // (rank discovered check by rook/queen only; EP evasion only when the captured
@@ -72,28 +77,80 @@ static constexpr u64 DIAG_A8H1 = 0x0102040810204080ULL;
 
 PG_HD int popc(u64 x) { return __builtin_popcountll(x); }
 PG_HD int lsb(u64 x) { return __builtin_ctzll(x); }
-PG_HD u64 rev64(u64 x) { return __builtin_bitreverse64(x); }
-PG_HD u64 sel(bool c, u64 a, u64 b) { return c ? a : b; }
 
+// ------------------------------------------------------------ 64-bit sets
+// A set of squares as two 32-bit halves.  Bitwise operators work per half,
+// so chains of them fuse into v_bitop3_b32 / v_and_or_b32 / v_or3_b32 (the
+// compiler keeps 64-bit bitwise operations whole and never fuses those);
+// add, subtract, popcount and find-first go through the 64-bit value (the
+// same v_add/sub_co pairs, v_bcnt and v_ffbl either way).  Built with
+// -fno-slp-vectorize so the halves are not merged back into 64-bit vectors.
+struct BB {
+    u32 lo, hi;
+    PG_HD constexpr BB() : lo(0), hi(0) {}
+    PG_HD constexpr BB(u64 x) : lo((u32)x), hi((u32)(x >> 32)) {}
+    PG_HD constexpr BB(u32 l, u32 h, int) : lo(l), hi(h) {}
+    PG_HD constexpr u64 v() const { return ((u64)hi << 32) | lo; }
+    PG_HD constexpr explicit operator bool() const { return v() != 0; }
+};
+PG_HD constexpr BB operator&(BB a, BB b) { return BB(a.lo & b.lo, a.hi & b.hi, 0); }
+PG_HD constexpr BB operator|(BB a, BB b) { return BB(a.lo | b.lo, a.hi | b.hi, 0); }
+PG_HD constexpr BB operator^(BB a, BB b) { return BB(a.lo ^ b.lo, a.hi ^ b.hi, 0); }
+PG_HD constexpr BB operator~(BB a) { return BB(~a.lo, ~a.hi, 0); }
+PG_HD constexpr BB &operator&=(BB &a, BB b) { return a = a & b; }
+PG_HD constexpr BB &operator|=(BB &a, BB b) { return a = a | b; }
+PG_HD constexpr BB &operator^=(BB &a, BB b) { return a = a ^ b; }
+PG_HD constexpr BB operator-(BB a, BB b) { return BB(a.v() - b.v()); }
+PG_HD constexpr bool operator==(BB a, BB b) { return a.v() == b.v(); }
+PG_HD constexpr bool operator!=(BB a, BB b) { return a.v() != b.v(); }
+PG_HD constexpr bool operator!(BB a) { return a.v() == 0; }
+PG_HD int popc(BB x) { return __builtin_popcountll(x.v()); }
+PG_HD int lsb(BB x) { return __builtin_ctzll(x.v()); }
+PG_HD BB lowBit(BB x) { return BB(x.v() & (0 - x.v())); }
+PG_HD BB clearLow(BB x) { return BB(x.v() & (x.v() - 1)); }
+PG_HD BB bit(int sq) { return BB(1ULL << sq); }
+PG_HD BB rev64(BB x) { return BB(__builtin_bitreverse32(x.hi), __builtin_bitreverse32(x.lo), 0); }
+
+// Board shift by S (positive: towards h8).  On the device one 64-bit shift
+// instruction; a shift by 32 is a move of halves.
 template <int S>
-PG_HD u64 shift(u64 b)
+PG_HD constexpr u64 shift(u64 b)
 {
     if constexpr (S > 0) return b << S;
     else return b >> (-S);
 }
+template <int S>
+PG_HD BB shift(BB b)
+{
+    static_assert(S != 0 && S > -64 && S < 64, "shift amount");
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (S == 32) {
+        return BB(0u, b.lo, 0);
+    } else if constexpr (S == -32) {
+        return BB(b.hi, 0u, 0);
+    } else {
+        u64 r;
+        if constexpr (S > 0) asm("v_lshlrev_b64 %0, %2, %1" : "=v"(r) : "v"(b.v()), "n"(S));
+        else asm("v_lshrrev_b64 %0, %2, %1" : "=v"(r) : "v"(b.v()), "n"(-S));
+        return BB(r);
+    }
+#else
+    return BB(shift<S>(b.v()));
+#endif
+}
 
 // ---------------------------------------------------------------- piece sets
 struct Sets {
-    u64 occ, us, them, pawns, knights, diag, orth, kings;
+    BB occ, us, them, pawns, knights, diag, orth, kings;
 };
 
 template <int C>
 PG_HD Sets derive(const QuadBitBoard &q)
 {
     Sets s;
-    u64 b1 = q.bb[1], b2 = q.bb[2], b3 = q.bb[3];
+    BB b1 = q.bb[1], b2 = q.bb[2], b3 = q.bb[3];
     s.occ = b1 | b2 | b3;
-    u64 black = q.bb[0] & s.occ;
+    BB black = BB(q.bb[0]) & s.occ;
     s.us = C == WHITE ? (s.occ ^ black) : black;
     s.them = s.occ ^ s.us;
     s.pawns = b1 & ~(b2 | b3);
@@ -110,35 +167,97 @@ PG_HD int pieceAt(const QuadBitBoard &q, int sq)
 }
 
 // ------------------------------------------------------------ leaper attacks
-PG_HD u64 knightAttacks(u64 n)
+// Written once for plain 64-bit words (compile-time tables) and for BB.
+template <class T>
+PG_HD constexpr T knightAttacks(T n)
 {
-    u64 l1 = (n >> 1) & NOT_H, l2 = (n >> 2) & NOT_GH;
-    u64 r1 = (n << 1) & NOT_A, r2 = (n << 2) & NOT_AB;
-    u64 h1 = l1 | r1, h2 = l2 | r2;
-    return (h1 << 16) | (h1 >> 16) | (h2 << 8) | (h2 >> 8);
+    T l1 = shift<-1>(n) & NOT_H, l2 = shift<-2>(n) & NOT_GH;
+    T r1 = shift<1>(n) & NOT_A, r2 = shift<2>(n) & NOT_AB;
+    T h1 = l1 | r1, h2 = l2 | r2;
+    return shift<16>(h1) | shift<-16>(h1) | shift<8>(h2) | shift<-8>(h2);
 }
 
-PG_HD u64 kingAttacks(u64 k)
+template <class T>
+PG_HD constexpr T kingAttacks(T k)
 {
-    u64 a = ((k << 1) & NOT_A) | ((k >> 1) & NOT_H);
-    u64 r = k | a;
-    return a | (r << 8) | (r >> 8);
+    T a = (shift<1>(k) & NOT_A) | (shift<-1>(k) & NOT_H);
+    T r = k | a;
+    return a | shift<8>(r) | shift<-8>(r);
 }
 
-template <int C>  // squares attacked by pawns of colour C
-PG_HD u64 pawnAttacks(u64 p)
+template <int C, class T>  // squares attacked by pawns of colour C
+PG_HD constexpr T pawnAttacks(T p)
 {
-    if constexpr (C == WHITE) return ((p << 9) & NOT_A) | ((p << 7) & NOT_H);
-    else return ((p >> 7) & NOT_A) | ((p >> 9) & NOT_H);
+    if constexpr (C == WHITE) return (shift<9>(p) & NOT_A) | (shift<7>(p) & NOT_H);
+    else return (shift<-7>(p) & NOT_A) | (shift<-9>(p) & NOT_H);
+}
+
+// ------------------------------------------------------ per-square masks
+// Line masks through `sq`, excluding sq itself.
+PG_HD constexpr u64 fileMaskEx(int sq) { return (FILE_A << (sq & 7)) ^ (1ULL << sq); }
+PG_HD constexpr u64 rankMaskEx(int sq) { return (0xFFULL << (sq & 56)) ^ (1ULL << sq); }
+PG_HD constexpr u64 diagMaskEx(int sq)  // a1-h8 direction (NE/SW)
+{
+    int d = (sq >> 3) - (sq & 7);  // rank - file
+    u64 m = d >= 0 ? (DIAG_A1H8 << (8 * d)) : (DIAG_A1H8 >> (-8 * d));
+    return m ^ (1ULL << sq);
+}
+PG_HD constexpr u64 antiMaskEx(int sq)  // a8-h1 direction (NW/SE)
+{
+    int d = (sq >> 3) + (sq & 7) - 7;
+    u64 m = d >= 0 ? (DIAG_A8H1 << (8 * d)) : (DIAG_A8H1 >> (-8 * d));
+    return m ^ (1ULL << sq);
+}
+
+// One 64-byte row per square, built at compile time: the four lines through
+// the square (square excluded), knight and king attacks, and pawn attacks by
+// colour.  On the device the table lives in constant memory (4 KB, cached);
+// a countMoves reads the king's row (lines, king, knight and pawn attacks),
+// the enemy king's king attacks and one row per own knight.
+struct SqMasks {
+    u64 file, rank, diag, anti, knight, king, pawn[2];
+};
+struct SqTable {
+    SqMasks t[64];
+};
+constexpr SqTable buildSqTable()
+{
+    SqTable T{};
+    for (int sq = 0; sq < 64; sq++) {
+        u64 b = 1ULL << sq;
+        SqMasks &m = T.t[sq];
+        m.file = fileMaskEx(sq);
+        m.rank = rankMaskEx(sq);
+        m.diag = diagMaskEx(sq);
+        m.anti = antiMaskEx(sq);
+        m.knight = knightAttacks<u64>(b);
+        m.king = kingAttacks<u64>(b);
+        m.pawn[WHITE] = pawnAttacks<WHITE, u64>(b);
+        m.pawn[BLACK] = pawnAttacks<BLACK, u64>(b);
+    }
+    return T;
+}
+#if defined(__HIPCC__)
+static __constant__ const SqTable kSqDev = buildSqTable();
+#endif
+static constexpr SqTable kSqHost = buildSqTable();
+
+PG_HD const SqMasks &sqMasks(int sq)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return kSqDev.t[sq];
+#else
+    return kSqHost.t[sq];
+#endif
 }
 
 // ------------------------------------------------------- Kogge-Stone fills
 // Squares attacked along direction S (board shift) by every slider in `gen`,
 // given the set of empty (transparent) squares.  WRAP masks file wrap-around.
 template <int S, u64 WRAP>
-PG_HD u64 rayAttacks(u64 gen, u64 empty)
+PG_HD BB rayAttacks(BB gen, BB empty)
 {
-    u64 pro = empty & WRAP;
+    BB pro = empty & WRAP;
     gen |= pro & shift<S>(gen);
     pro &= shift<S>(pro);
     gen |= pro & shift<2 * S>(gen);
@@ -150,63 +269,48 @@ PG_HD u64 rayAttacks(u64 gen, u64 empty)
 // East attacks of every slider in s, rank-wise subtraction (o ^ (o - 2s)):
 // the h-file is treated as occupied so a borrow never leaves its rank, and
 // h-file sliders (no east moves) are dropped.
-PG_HD u64 eastAttacks(u64 s, u64 occ)
+PG_HD BB eastAttacks(BB s, BB occ)
 {
-    u64 o = occ | FILE_H;
-    u64 g = s & NOT_H;
-    return o ^ (o - (g << 1));
+    BB o = occ | FILE_H;
+    BB g = s & NOT_H;
+    return o ^ (o - shift<1>(g));
 }
 // West attacks: east attacks in the bit-reversed board
-PG_HD u64 westAttacks(u64 s, u64 occ) { return rev64(eastAttacks(rev64(s), rev64(occ))); }
+PG_HD BB westAttacks(BB s, BB occ) { return rev64(eastAttacks(rev64(s), rev64(occ))); }
 
-PG_HD u64 orthAttacksSet(u64 s, u64 empty)
+PG_HD BB orthAttacksSet(BB s, BB empty)
 {
     return rayAttacks<8, ~0ULL>(s, empty) | rayAttacks<-8, ~0ULL>(s, empty) | eastAttacks(s, ~empty) |
            westAttacks(s, ~empty);
 }
-PG_HD u64 diagAttacksSet(u64 s, u64 empty)
+PG_HD BB diagAttacksSet(BB s, BB empty)
 {
-    return rayAttacks<9, NOT_A>(s, empty) | rayAttacks<7, NOT_H>(s, empty) |
-           rayAttacks<-7, NOT_A>(s, empty) | rayAttacks<-9, NOT_H>(s, empty);
-}
-
-// ------------------------------------------------- single-square line attacks
-// Line masks through `sq`, excluding sq itself.
-PG_HD u64 fileMaskEx(int sq) { return (FILE_A << (sq & 7)) ^ (1ULL << sq); }
-PG_HD u64 rankMaskEx(int sq) { return (0xFFULL << (sq & 56)) ^ (1ULL << sq); }
-PG_HD u64 diagMaskEx(int sq)  // a1-h8 direction (NE/SW)
-{
-    int d = (sq >> 3) - (sq & 7);  // rank - file
-    u64 m = d >= 0 ? (DIAG_A1H8 << (8 * d)) : (DIAG_A1H8 >> (-8 * d));
-    return m ^ (1ULL << sq);
-}
-PG_HD u64 antiMaskEx(int sq)  // a8-h1 direction (NW/SE)
-{
-    int d = (sq >> 3) + (sq & 7) - 7;
-    u64 m = d >= 0 ? (DIAG_A8H1 << (8 * d)) : (DIAG_A8H1 >> (-8 * d));
-    return m ^ (1ULL << sq);
+    return rayAttacks<9, NOT_A>(s, empty) | rayAttacks<7, NOT_H>(s, empty) | rayAttacks<-7, NOT_A>(s, empty) |
+           rayAttacks<-9, NOT_H>(s, empty);
 }
 
 // Hyperbola quintessence with a full bit reverse: attacks from the square with
 // bit `b` along the line `m` (mask excluding the square).  Works for ranks too
 // because reversing all 64 bits reverses the order inside each rank.
-PG_HD u64 lineAttacks(u64 occ, u64 m, u64 b)
+PG_HD BB lineAttacks(BB occ, BB m, BB b)
 {
-    u64 o = occ & m;
-    u64 f = o - b;
-    u64 r = rev64(rev64(o) - rev64(b));
+    BB o = occ & m;
+    BB f = o - b;
+    BB r = rev64(rev64(o) - rev64(b));
     return (f ^ r) & m;
 }
 
-PG_HD u64 rookAttacks(int sq, u64 occ)
+PG_HD BB rookAttacks(int sq, BB occ)
 {
-    u64 b = 1ULL << sq;
-    return lineAttacks(occ, fileMaskEx(sq), b) | lineAttacks(occ, rankMaskEx(sq), b);
+    const SqMasks &M = sqMasks(sq);
+    BB b = bit(sq);
+    return lineAttacks(occ, M.file, b) | lineAttacks(occ, M.rank, b);
 }
-PG_HD u64 bishopAttacks(int sq, u64 occ)
+PG_HD BB bishopAttacks(int sq, BB occ)
 {
-    u64 b = 1ULL << sq;
-    return lineAttacks(occ, diagMaskEx(sq), b) | lineAttacks(occ, antiMaskEx(sq), b);
+    const SqMasks &M = sqMasks(sq);
+    BB b = bit(sq);
+    return lineAttacks(occ, M.diag, b) | lineAttacks(occ, M.anti, b);
 }
 
 // ------------------------------------------------------------ legality frame
@@ -214,12 +318,13 @@ PG_HD u64 bishopAttacks(int sq, u64 occ)
 struct Frame {
     Sets s;
     int k;           // own king square
-    u64 kBit;
-    u64 att;         // squares attacked by the enemy (x-ray through the king on check lines)
-    u64 checkers;
-    u64 checkMask;   // landing squares that resolve the check (all if none, none if double)
-    u64 pinFile, pinRank, pinDiag, pinAnti;  // pin rays (king..pinner] per line
-    u64 pinned;      // own pieces on any pin ray
+    BB kBit;
+    BB kAtt;         // king-move squares from the own king square
+    BB att;          // squares attacked by the enemy (x-ray through the king on check lines)
+    BB checkers;
+    BB checkMask;    // landing squares that resolve the check (all if none, none if double)
+    BB pinFile, pinRank, pinDiag, pinAnti;  // pin rays (king..pinner] per line
+    BB pinned;       // own pieces on any pin ray
     int nCheck;
 };
 
@@ -239,40 +344,46 @@ PG_HD Frame framePins(const QuadBitBoard &q)
     f.kBit = s.kings & s.us;
     f.k = lsb(f.kBit);
     const int k = f.k;
-    u64 eDiag = s.diag & s.them, eOrth = s.orth & s.them;
-    u64 mFile = fileMaskEx(k), mRank = rankMaskEx(k), mDiag = diagMaskEx(k), mAnti = antiMaskEx(k);
-    u64 snipers = ((mFile | mRank) & eOrth) | ((mDiag | mAnti) & eDiag);
-    u64 below = f.kBit - 1, above = ~below & ~f.kBit;
-    u64 kAtt = kingAttacks(f.kBit);
-    u64 chkRays = 0, behind = 0, pinAll = 0;
+    const SqMasks &K = sqMasks(k);
+    BB eDiag = s.diag & s.them, eOrth = s.orth & s.them;
+    BB mFile = K.file, mRank = K.rank, mDiag = K.diag, mAnti = K.anti;
+    BB snipers = ((mFile | mRank) & eOrth) | ((mDiag | mAnti) & eDiag);
+    BB below = f.kBit - BB(1), above = ~below & ~f.kBit;
+    f.kAtt = K.king;
+    BB chkRays, behind, pinAll;
     while (snipers) {
-        u64 sb = snipers & (0 - snipers);
+        BB sb = lowBit(snipers);
         snipers ^= sb;
         int sq = lsb(sb);
-        u64 line = (mFile & sb) ? mFile : ((mRank & sb) ? mRank : ((mDiag & sb) ? mDiag : mAnti));
+        // the sniper's line, selected per 32-bit half (a select of whole BB
+        // values here is lowered to an indexed private array)
+        bool inF = (bool)(mFile & sb), inR = (bool)(mRank & sb), inD = (bool)(mDiag & sb);
+        BB line(inF ? mFile.lo : inR ? mRank.lo : inD ? mDiag.lo : mAnti.lo,
+                inF ? mFile.hi : inR ? mRank.hi : inD ? mDiag.hi : mAnti.hi, 0);
         bool up = sq > k;
-        u64 btw = line & (up ? (sb - (f.kBit << 1)) : (f.kBit - (sb << 1)));
-        u64 blk = btw & s.occ;
-        u64 ray = btw | sb;
-        bool check = blk == 0;
-        bool pin = blk && !(blk & (blk - 1)) && (blk & s.us);
-        chkRays |= check ? ray : 0;
-        behind |= check ? (kAtt & line & (up ? below : above)) : 0;
-        pinAll |= pin ? ray : 0;
+        BB btw = line & (up ? (sb - shift<1>(f.kBit)) : (f.kBit - shift<1>(sb)));
+        BB blk = btw & s.occ;
+        BB ray = btw | sb;
+        bool check = !blk;
+        bool pin = blk && !clearLow(blk) && (blk & s.us);
+        chkRays |= check ? ray : BB();
+        behind |= check ? (f.kAtt & line & (up ? below : above)) : BB();
+        pinAll |= pin ? ray : BB();
     }
     // pin rays of different lines meet only at the king, so the lines split them
     f.pinFile = pinAll & mFile;
     f.pinRank = pinAll & mRank;
     f.pinDiag = pinAll & mDiag;
     f.pinAnti = pinAll & mAnti;
-    u64 leaperCheckers = (knightAttacks(f.kBit) & s.knights & s.them) | (pawnAttacks<C>(f.kBit) & s.pawns & s.them);
+    BB leaperCheckers = (BB(K.knight) & s.knights & s.them) | (BB(K.pawn[C]) & s.pawns & s.them);
     f.checkers = leaperCheckers | (chkRays & (eOrth | eDiag));
     f.nCheck = popc(f.checkers);
-    f.checkMask = f.nCheck == 0 ? ~0ULL : (f.nCheck == 1 ? (chkRays | leaperCheckers) : 0ULL);
+    f.checkMask = f.nCheck == 0 ? BB(~0ULL) : (f.nCheck == 1 ? (chkRays | leaperCheckers) : BB());
     f.pinned = (f.pinFile | f.pinRank | f.pinDiag | f.pinAnti) & s.us;
     // leaper and king attacks here; slider attacks are added by the caller
-    f.att = pawnAttacks<1 - C>(s.pawns & s.them) | knightAttacks(s.knights & s.them) | kingAttacks(s.kings & s.them) |
-            behind;
+    BB eK = s.kings & s.them;
+    BB eKAtt = eK ? BB(sqMasks(lsb(eK)).king) : BB();
+    f.att = pawnAttacks<1 - C>(s.pawns & s.them) | knightAttacks(s.knights & s.them) | eKAtt | behind;
     return f;
 }
 
@@ -280,7 +391,7 @@ template <int C>
 PG_HD Frame frame(const QuadBitBoard &q)
 {
     Frame f = framePins<C>(q);
-    u64 empty = ~f.s.occ;
+    BB empty = ~f.s.occ;
     f.att |= orthAttacksSet(f.s.orth & f.s.them, empty) | diagAttacksSet(f.s.diag & f.s.them, empty);
     return f;
 }
@@ -289,11 +400,11 @@ PG_HD Frame frame(const QuadBitBoard &q)
 // squares: the propagator chain (empty & wrap, shifted and-ed twice) is
 // computed once and shared by the enemy-attack fill and our move fill.
 template <int S, u64 WRAP>
-PG_HD void rayPair(u64 enemy, u64 own, u64 empty, u64 &att, u64 &ownAtt)
+PG_HD void rayPair(BB enemy, BB own, BB empty, BB &att, BB &ownAtt)
 {
-    u64 p0 = empty & WRAP;
-    u64 p1 = p0 & shift<S>(p0);
-    u64 p2 = p1 & shift<2 * S>(p1);
+    BB p0 = empty & WRAP;
+    BB p1 = p0 & shift<S>(p0);
+    BB p2 = p1 & shift<2 * S>(p1);
     enemy |= p0 & shift<S>(enemy);
     own |= p0 & shift<S>(own);
     enemy |= p1 & shift<2 * S>(enemy);
@@ -307,32 +418,32 @@ PG_HD void rayPair(u64 enemy, u64 own, u64 empty, u64 &att, u64 &ownAtt)
 // En-passant moves, reference semantics (MoveGeneratorBitboard.h:1732-1770,
 // 1639-1650).  Returns a mask of legal EP source pawns.
 template <int C>
-PG_HD u64 epSources(const Frame &f, u8 state)
+PG_HD BB epSources(const Frame &f, u8 state)
 {
     int ep = state >> 4;
-    if (!ep) return 0;
+    if (!ep) return BB();
     const Sets &s = f.s;
-    u64 target = 1ULL << ((ep - 1) + (C == WHITE ? 40 : 16));
-    u64 capSq = C == WHITE ? (target >> 8) : (target << 8);
-    u64 myPawns = s.pawns & s.us;
-    u64 src = (((capSq << 1) & NOT_A) | ((capSq >> 1) & NOT_H)) & myPawns;
-    if (!src) return 0;
+    BB target = bit((ep - 1) + (C == WHITE ? 40 : 16));
+    BB capSq = C == WHITE ? shift<-8>(target) : shift<8>(target);
+    BB myPawns = s.pawns & s.us;
+    BB src = ((shift<1>(capSq) & NOT_A) | (shift<-1>(capSq) & NOT_H)) & myPawns;
+    if (!src) return BB();
     if (f.nCheck) {
         // only when the captured pawn is the single checker, unpinned pawns only
-        return (f.nCheck == 1 && capSq == f.checkers) ? (src & ~f.pinned) : 0;
+        return (f.nCheck == 1 && capSq == f.checkers) ? (src & ~f.pinned) : BB();
     }
-    u64 legal = 0;
+    BB legal;
     while (src) {
-        u64 p = src & (0 - src);
+        BB p = lowBit(src);
         src ^= p;
         if (p & f.pinned) {
             // a pinned pawn may capture along its pin line only
-            u64 line = (p & f.pinDiag) ? f.pinDiag : ((p & f.pinAnti) ? f.pinAnti : 0);
+            BB line = (p & f.pinDiag) ? f.pinDiag : ((p & f.pinAnti) ? f.pinAnti : BB());
             if (target & line) legal |= p;
         } else {
             // removing both pawns must not open the king's rank to a rook/queen
-            u64 occ2 = s.occ ^ capSq ^ p;
-            u64 r = lineAttacks(occ2, rankMaskEx(f.k), f.kBit);
+            BB occ2 = s.occ ^ capSq ^ p;
+            BB r = lineAttacks(occ2, sqMasks(f.k).rank, f.kBit);
             if (!(r & s.orth & s.them)) legal |= p;
         }
     }
@@ -340,26 +451,26 @@ PG_HD u64 epSources(const Frame &f, u8 state)
 }
 
 // ------------------------------------------------------------- move counting
-// Number of legal moves (perft(1)).  Set-wise, no per-piece loops: the eight
-// slider directions are filled once for the enemy's attacks and our moves
-// together; knights and pawns are counted by shifted sets.
+// Number of legal moves (perft(1)).  Set-wise: the eight slider directions
+// are filled once for the enemy's attacks and our moves together; pawns are
+// counted by shifted sets, knights by one table row each.
 template <int C>
 PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
 {
     Frame f = framePins<C>(q);
     const Sets &s = f.s;
-    u64 empty = ~s.occ;
-    u64 targets = ~s.us & f.checkMask;
-    u64 freeP = ~f.pinned;
+    BB empty = ~s.occ;
+    BB targets = ~s.us & f.checkMask;
+    BB freeP = ~f.pinned;
 
     // sliders: a direction's fill reaches each target from exactly one slider,
     // so per-direction popcounts count moves exactly; a pinned slider moves
     // only along its pin line
-    u64 D = s.diag & s.us, O = s.orth & s.us;
-    u64 eD = s.diag & s.them, eO = s.orth & s.them;
-    u64 dA1H8 = D & (freeP | f.pinDiag), dA8H1 = D & (freeP | f.pinAnti);
-    u64 oFile = O & (freeP | f.pinFile), oRank = O & (freeP | f.pinRank);
-    u64 att = f.att, r;
+    BB D = s.diag & s.us, O = s.orth & s.us;
+    BB eD = s.diag & s.them, eO = s.orth & s.them;
+    BB dA1H8 = D & (freeP | f.pinDiag), dA8H1 = D & (freeP | f.pinAnti);
+    BB oFile = O & (freeP | f.pinFile), oRank = O & (freeP | f.pinRank);
+    BB att = f.att, r;
     u32 n = 0;
     rayPair<8, ~0ULL>(eO, oFile, empty, att, r);
     n += popc(r & targets);
@@ -376,30 +487,23 @@ PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
     rayPair<-7, NOT_A>(eD, dA8H1, empty, att, r);
     n += popc(r & targets);
 
-    n += popc(kingAttacks(f.kBit) & ~s.us & ~att);
+    n += popc(f.kAtt & ~s.us & ~att);
 
     // knights: pinned knights never move
-    u64 N = s.knights & s.us & freeP;
-    {
-        u64 l1 = (N >> 1) & NOT_H, l2 = (N >> 2) & NOT_GH;
-        u64 r1 = (N << 1) & NOT_A, r2 = (N << 2) & NOT_AB;
-        n += popc((l1 << 16) & targets) + popc((l1 >> 16) & targets) + popc((r1 << 16) & targets) +
-             popc((r1 >> 16) & targets) + popc((l2 << 8) & targets) + popc((l2 >> 8) & targets) +
-             popc((r2 << 8) & targets) + popc((r2 >> 8) & targets);
-    }
+    for (BB N = s.knights & s.us & freeP; N; N = clearLow(N)) n += popc(BB(sqMasks(lsb(N)).knight) & targets);
 
     // pawns
-    u64 P = s.pawns & s.us;
-    u64 promo = C == WHITE ? RANK_8 : RANK_1;
-    u64 push = P & (freeP | f.pinFile);
-    u64 s1 = (C == WHITE ? push << 8 : push >> 8) & empty;
-    u64 s2 = (C == WHITE ? (s1 & RANK_3) << 8 : (s1 & RANK_6) >> 8) & empty & f.checkMask;
+    BB P = s.pawns & s.us;
+    const u64 promo = C == WHITE ? RANK_8 : RANK_1;
+    BB push = P & (freeP | f.pinFile);
+    BB s1 = (C == WHITE ? shift<8>(push) : shift<-8>(push)) & empty;
+    BB s2 = (C == WHITE ? shift<8>(s1 & RANK_3) : shift<-8>(s1 & RANK_6)) & empty & f.checkMask;
     s1 &= f.checkMask;
     // capture towards the h-file: white NE (+9, a1-h8 line), black SE (-7, a8-h1 line)
-    u64 pe = P & (freeP | (C == WHITE ? f.pinDiag : f.pinAnti));
-    u64 pw = P & (freeP | (C == WHITE ? f.pinAnti : f.pinDiag));
-    u64 ce = (C == WHITE ? (pe << 9) : (pe >> 7)) & NOT_A & s.them & f.checkMask;
-    u64 cw = (C == WHITE ? (pw << 7) : (pw >> 9)) & NOT_H & s.them & f.checkMask;
+    BB pe = P & (freeP | (C == WHITE ? f.pinDiag : f.pinAnti));
+    BB pw = P & (freeP | (C == WHITE ? f.pinAnti : f.pinDiag));
+    BB ce = (C == WHITE ? shift<9>(pe) : shift<-7>(pe)) & NOT_A & s.them & f.checkMask;
+    BB cw = (C == WHITE ? shift<7>(pw) : shift<-9>(pw)) & NOT_H & s.them & f.checkMask;
     n += popc(s1 & ~promo) + popc(s2) + popc(ce & ~promo) + popc(cw & ~promo) +
          4 * (popc(s1 & promo) + popc(ce & promo) + popc(cw & promo));
 
@@ -432,10 +536,10 @@ PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
 {
     Frame f = frame<C>(q);
     const Sets &s = f.s;
-    u64 empty = ~s.occ;
-    u64 targets = ~s.us & f.checkMask;
+    BB empty = ~s.occ;
+    BB targets = ~s.us & f.checkMask;
     u32 n = 0;
-    for (u64 t = kingAttacks(f.kBit) & ~s.us & ~f.att; t; t &= t - 1) {
+    for (BB t = f.kAtt & ~s.us & ~f.att; t; t = clearLow(t)) {
         emit(n, (u16)(f.k | (lsb(t) << 6)));
         n++;
     }
@@ -453,42 +557,42 @@ PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
             n++;
         }
     }
-    u64 freeP = ~f.pinned;
-    for (u64 b = s.knights & s.us & freeP; b; b &= b - 1) {
+    BB freeP = ~f.pinned;
+    for (BB b = s.knights & s.us & freeP; b; b = clearLow(b)) {
         int from = lsb(b);
-        for (u64 t = knightAttacks(1ULL << from) & targets; t; t &= t - 1) {
+        for (BB t = BB(sqMasks(from).knight) & targets; t; t = clearLow(t)) {
             emit(n, (u16)(from | (lsb(t) << 6)));
             n++;
         }
     }
-    for (u64 b = s.diag & s.us & ~(f.pinFile | f.pinRank); b; b &= b - 1) {
+    for (BB b = s.diag & s.us & ~(f.pinFile | f.pinRank); b; b = clearLow(b)) {
         int from = lsb(b);
-        u64 fb = 1ULL << from;
-        u64 t = bishopAttacks(from, s.occ) & targets;
-        t &= (fb & f.pinDiag) ? f.pinDiag : ((fb & f.pinAnti) ? f.pinAnti : ~0ULL);
-        for (; t; t &= t - 1) {
+        BB fb = bit(from);
+        BB t = bishopAttacks(from, s.occ) & targets;
+        t &= (fb & f.pinDiag) ? f.pinDiag : ((fb & f.pinAnti) ? f.pinAnti : BB(~0ULL));
+        for (; t; t = clearLow(t)) {
             emit(n, (u16)(from | (lsb(t) << 6)));
             n++;
         }
     }
-    for (u64 b = s.orth & s.us & ~(f.pinDiag | f.pinAnti); b; b &= b - 1) {
+    for (BB b = s.orth & s.us & ~(f.pinDiag | f.pinAnti); b; b = clearLow(b)) {
         int from = lsb(b);
-        u64 fb = 1ULL << from;
-        u64 t = rookAttacks(from, s.occ) & targets;
-        t &= (fb & f.pinFile) ? f.pinFile : ((fb & f.pinRank) ? f.pinRank : ~0ULL);
-        for (; t; t &= t - 1) {
+        BB fb = bit(from);
+        BB t = rookAttacks(from, s.occ) & targets;
+        t &= (fb & f.pinFile) ? f.pinFile : ((fb & f.pinRank) ? f.pinRank : BB(~0ULL));
+        for (; t; t = clearLow(t)) {
             emit(n, (u16)(from | (lsb(t) << 6)));
             n++;
         }
     }
-    u64 P = s.pawns & s.us;
-    u64 promo = C == WHITE ? RANK_8 : RANK_1;
+    BB P = s.pawns & s.us;
+    const u64 promo = C == WHITE ? RANK_8 : RANK_1;
     const int fwd = C == WHITE ? 8 : -8;
-    u64 push = P & (freeP | f.pinFile);
-    u64 s1 = (C == WHITE ? push << 8 : push >> 8) & empty;
-    u64 s2 = (C == WHITE ? (s1 & RANK_3) << 8 : (s1 & RANK_6) >> 8) & empty & f.checkMask;
+    BB push = P & (freeP | f.pinFile);
+    BB s1 = (C == WHITE ? shift<8>(push) : shift<-8>(push)) & empty;
+    BB s2 = (C == WHITE ? shift<8>(s1 & RANK_3) : shift<-8>(s1 & RANK_6)) & empty & f.checkMask;
     s1 &= f.checkMask;
-    u64 ePawns = s.pawns & s.them;
+    BB ePawns = s.pawns & s.them;
     auto emitPawn = [&](int from, int to) {
         if ((1ULL << to) & promo) {
             // promotion order N, B, Q, R (MoveGeneratorBitboard.h:920-923)
@@ -502,33 +606,33 @@ PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
             n++;
         }
     };
-    for (u64 t = s1; t; t &= t - 1) {
+    for (BB t = s1; t; t = clearLow(t)) {
         int to = lsb(t);
         emitPawn(to - fwd, to);
     }
-    for (u64 t = s2; t; t &= t - 1) {
+    for (BB t = s2; t; t = clearLow(t)) {
         int to = lsb(t);
-        u64 tb = 1ULL << to;
-        bool epPossible = ((((tb << 1) & NOT_A) | ((tb >> 1) & NOT_H)) & ePawns) != 0;
+        BB tb = bit(to);
+        bool epPossible = (bool)(((shift<1>(tb) & NOT_A) | (shift<-1>(tb) & NOT_H)) & ePawns);
         emit(n, (u16)((to - 2 * fwd) | (to << 6) | ((epPossible ? MF_DPUSH : MF_QUIET) << 12)));
         n++;
     }
-    u64 pe = P & (freeP | (C == WHITE ? f.pinDiag : f.pinAnti));
-    u64 pw = P & (freeP | (C == WHITE ? f.pinAnti : f.pinDiag));
+    BB pe = P & (freeP | (C == WHITE ? f.pinDiag : f.pinAnti));
+    BB pw = P & (freeP | (C == WHITE ? f.pinAnti : f.pinDiag));
     const int de = C == WHITE ? 9 : -7, dw = C == WHITE ? 7 : -9;
-    u64 ce = (C == WHITE ? (pe << 9) : (pe >> 7)) & NOT_A & s.them & f.checkMask;
-    u64 cw = (C == WHITE ? (pw << 7) : (pw >> 9)) & NOT_H & s.them & f.checkMask;
-    for (u64 t = ce; t; t &= t - 1) {
+    BB ce = (C == WHITE ? shift<9>(pe) : shift<-7>(pe)) & NOT_A & s.them & f.checkMask;
+    BB cw = (C == WHITE ? shift<7>(pw) : shift<-9>(pw)) & NOT_H & s.them & f.checkMask;
+    for (BB t = ce; t; t = clearLow(t)) {
         int to = lsb(t);
         emitPawn(to - de, to);
     }
-    for (u64 t = cw; t; t &= t - 1) {
+    for (BB t = cw; t; t = clearLow(t)) {
         int to = lsb(t);
         emitPawn(to - dw, to);
     }
     if (state >> 4) {
         int to = ((state >> 4) - 1) + (C == WHITE ? 40 : 16);
-        for (u64 b = epSources<C>(f, state); b; b &= b - 1) {
+        for (BB b = epSources<C>(f, state); b; b = clearLow(b)) {
             emit(n, (u16)(lsb(b) | (to << 6) | (MF_EP << 12)));
             n++;
         }
@@ -560,23 +664,24 @@ template <int C>
 PG_HD void makeMove(QuadBitBoard &q, u8 &state, u16 m)
 {
     u32 from = m & 63, to = (m >> 6) & 63, flags = m >> 12;
-    u64 fb = 1ULL << from, tb = 1ULL << to;
+    u64 fb64 = 1ULL << from, tb64 = 1ULL << to;
+    BB fb = fb64, tb = tb64;
     u32 piece = (u32)(((q.bb[1] >> from) & 1) | (((q.bb[2] >> from) & 1) << 1) | (((q.bb[3] >> from) & 1) << 2));
     piece = (flags & MF_PROMO) ? (flags & 3) + KNIGHT : piece;
-    u64 clr = ~(fb | tb);
     // en passant: the captured pawn sits behind the target
-    u64 epCap = (flags == MF_EP) ? (C == WHITE ? tb >> 8 : tb << 8) : 0;
+    BB epCap = (flags == MF_EP) ? (C == WHITE ? shift<-8>(tb) : shift<8>(tb)) : BB();
     // castling: the rook jumps to the square the king passes
     const u32 base = C == WHITE ? 0 : 56;
-    u64 rookFrom = flags == MF_KCASTLE ? (1ULL << (base + 7)) : (flags == MF_QCASTLE ? (1ULL << base) : 0);
-    u64 rookTo = flags == MF_KCASTLE ? (1ULL << (base + 5)) : (flags == MF_QCASTLE ? (1ULL << (base + 3)) : 0);
-    clr &= ~(epCap | rookFrom);
-    q.bb[0] = (q.bb[0] & clr) | (C == BLACK ? (tb | rookTo) : 0);
-    q.bb[1] = (q.bb[1] & clr) | ((piece & 1) ? tb : 0);
-    q.bb[2] = (q.bb[2] & clr) | ((piece & 2) ? tb : 0);
-    q.bb[3] = (q.bb[3] & clr) | ((piece & 4) ? tb : 0) | rookTo;
+    BB rookFrom = flags == MF_KCASTLE ? BB(1ULL << (base + 7)) : (flags == MF_QCASTLE ? BB(1ULL << base) : BB());
+    BB rookTo = flags == MF_KCASTLE ? BB(1ULL << (base + 5)) : (flags == MF_QCASTLE ? BB(1ULL << (base + 3)) : BB());
+    BB clr = ~(fb | tb | epCap | rookFrom);
+    BB b0 = q.bb[0], b1 = q.bb[1], b2 = q.bb[2], b3 = q.bb[3];
+    q.bb[0] = ((b0 & clr) | (C == BLACK ? (tb | rookTo) : BB())).v();
+    q.bb[1] = ((b1 & clr) | ((piece & 1) ? tb : BB())).v();
+    q.bb[2] = ((b2 & clr) | ((piece & 2) ? tb : BB())).v();
+    q.bb[3] = ((b3 & clr) | ((piece & 4) ? tb : BB()) | rookTo).v();
 
-    u64 touch = fb | tb;
+    u64 touch = fb64 | tb64;
     u32 c = 0;
     c |= (touch & (1ULL << 0)) ? 2u : 0u;
     c |= (touch & (1ULL << 7)) ? 1u : 0u;
