@@ -179,6 +179,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--leaf", type=int, default=0)
     ap.add_argument("--leaf2-variant", type=int, default=0)
+    ap.add_argument("--leaf-grid", type=int, default=0, help="leaf blocks per CU (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--serve", type=int, default=0)
@@ -199,6 +200,8 @@ def main():
     if a.leaf:
         pg.set_leaf_plies(a.leaf)
     pg.lib().pg_set_leaf2_variant(a.leaf2_variant)
+    if a.leaf_grid:
+        pg.lib().pg_set_leaf_grid(a.leaf_grid)
     pos = pg.parse_fen(fx["fen"])
     for i in range(a.warmup):
         c = pg.perft_bfs(pos, depth)

@@ -106,7 +106,7 @@ __device__ __forceinline__ void bttCount(BatchDev *d, bool probed, bool hit, boo
 __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
 {
     if (d->oom) return;
-    LvlDev &a = d->lv[0];
+    const LvlDev a = d->lv[0];
     u64 tiles = (a.n + BS - 1) / BS;
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u64 i = t * BS + threadIdx.x;
@@ -132,8 +132,8 @@ template <int C>  // side to move in the parent boards
 __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
 {
     if (d->oom) return;
-    LvlDev &a = d->lv[L];
-    const LvlDev &p = d->lv[L - 1];
+    const LvlDev a = d->lv[L];
+    const LvlDev p = d->lv[L - 1];
     u64 tiles = (a.n + BS - 1) / BS;
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u64 i = t * BS + threadIdx.x;
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
 __global__ __launch_bounds__(1024) void k_b_scan_top(BatchDev *d, int L)
 {
     if (d->oom) return;
-    LvlDev &a = d->lv[L];
+    const LvlDev a = d->lv[L];
     u64 tiles = (a.n + BS - 1) / BS;
     __shared__ u64 wtot[16];
     __shared__ u64 carry;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(1024) void k_b_scan_top(BatchDev *d, int L)
 __global__ __launch_bounds__(BS) void k_b_scan_apply(BatchDev *d, int L)
 {
     if (d->oom) return;
-    LvlDev &a = d->lv[L];
+    const LvlDev a = d->lv[L];
     u64 tiles = (a.n + BS - 1) / BS;
     __shared__ u32 wtot[BS / 64];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -239,7 +239,7 @@ __global__ void k_b_alloc(BatchDev *d, int L)
 __global__ __launch_bounds__(BS) void k_b_partitions(BatchDev *d, int L)
 {
     if (d->oom) return;
-    const LvlDev &a = d->lv[L];
+    const LvlDev a = d->lv[L];
     u64 n = a.n, c = d->lv[L + 1].n;
     if (!c) return;
     u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
@@ -254,8 +254,8 @@ __global__ __launch_bounds__(BS) void k_b_partitions(BatchDev *d, int L)
 __global__ __launch_bounds__(BS) void k_b_expand(BatchDev *d, int L)
 {
     if (d->oom) return;
-    const LvlDev &a = d->lv[L];
-    LvlDev &b = d->lv[L + 1];
+    const LvlDev a = d->lv[L];
+    const LvlDev b = d->lv[L + 1];
     u64 c = b.n;
     u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
     __shared__ u32 win[EXPAND_WCAP];
@@ -290,7 +290,7 @@ template <int C>  // side to move in this level's boards
 __global__ __launch_bounds__(BS) void k_b_genmoves(BatchDev *d, int L)
 {
     if (d->oom) return;
-    const LvlDev &a = d->lv[L];
+    const LvlDev a = d->lv[L];
     u16 *out = d->lv[L + 1].mv;
     u64 n = a.n;
     const u32 *incl = a.cnt;
@@ -343,8 +343,8 @@ __global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
 {
     if (d->oom) return;
     const u32 L = d->leafLevel;
-    const LvlDev &a = d->lv[L];
-    const LvlDev &p = d->lv[L - 1];
+    const LvlDev a = d->lv[L];
+    const LvlDev p = d->lv[L - 1];
     const u64 n = a.n;
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -368,8 +368,12 @@ __global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
 }
 
 // persistent fused two-ply leaf: the wave-cooperative scheme of k_leaf2
-// (child boards and the flat pair list in LDS) with work-queue chunks and
-// per-entry partial sums kept in LDS until the chunk is done
+// (child boards and the flat pair list in LDS) with work-queue chunks.  A
+// chunk whose entries all belong to one root (nearly all: roots own millions
+// of leaf entries and entries are root-ordered) sums its pair counts in
+// registers and adds one wave total to the root; a chunk that straddles roots
+// keeps per-entry partial sums in LDS (atomics) and attributes them per lane.
+// The choice is wave-uniform, so the pair loop does not diverge.
 constexpr int BL2_CAP = 2048;
 constexpr int BL2_WAVES = 4;
 template <int C>
@@ -381,9 +385,15 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
     __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
     __shared__ u32 sl[BL2_WAVES][BL2_CAP];
     const u32 L = d->leafLevel;
-    const LvlDev &a = d->lv[L];
-    const LvlDev &p = d->lv[L - 1];
-    const u64 n = a.n;
+    // descriptor fields into registers once (the atomics below would otherwise
+    // force them to be re-read from the descriptor)
+    const u32 *__restrict__ aIdx = d->lv[L].idx;
+    const u16 *__restrict__ aMv = d->lv[L].mv;
+    const u32 *__restrict__ aRoot = d->lv[L].root;
+    const QuadBitBoard *__restrict__ pBoards = d->lv[L - 1].boards;
+    const u8 *__restrict__ pStates = d->lv[L - 1].states;
+    u64 *rootCounts = d->rootCounts;
+    const u64 n = d->lv[L].n;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (;;) {
         u64 base = 0;
@@ -394,19 +404,22 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
         u64 e = base + lane;
         bool valid = lane < B_CHUNK && e < n;
         if (valid) {
-            u32 pi = a.idx[e];
-            QuadBitBoard q = p.boards[pi];
-            u8 st = p.states[pi];
-            makeMove<C>(q, st, a.mv[e]);
+            u32 pi = aIdx[e];
+            QuadBitBoard q = pBoards[pi];
+            u8 st = pStates[pi];
+            makeMove<C>(q, st, aMv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
             cnt = countMoves<1 - C>(q, st);
-            root = a.root[e];
+            root = aRoot[e];
         }
-        if (lane < B_CHUNK) sacc[w][lane] = 0;
+        const u32 r0 = __shfl(root, 0);  // lane 0 is always valid
+        const bool oneRoot = __all(valid ? root == r0 : true);
+        if (!oneRoot && lane < B_CHUNK) sacc[w][lane] = 0;
         u32 incl = waveInclScan(cnt);
         u32 off = incl - cnt;
         u32 total = __shfl(incl, 63);
+        u64 acc = 0;
         waveLdsSync();
         for (u32 wb = 0; wb < total; wb += BL2_CAP) {
             if (cnt && off < wb + BL2_CAP && off + cnt > wb) {
@@ -427,16 +440,22 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
                 makeMove<1 - C>(c, s2, (u16)x);
-                atomicAdd(&sacc[w][src], countMoves<C>(c, s2));  // LDS atomic
+                u32 cm = countMoves<C>(c, s2);
+                if (oneRoot) acc += cm;
+                else atomicAdd(&sacc[w][src], cm);  // LDS atomic
             }
             waveLdsSync();
         }
-        u64 v = valid ? sacc[w][lane] : 0;
-        addToRoots(d->rootCounts, valid, root, v);
+        if (oneRoot) {
+            u64 sum = waveSum64(acc);
+            if (lane == 0 && sum) atomicAdd((unsigned long long *)&rootCounts[r0], (unsigned long long)sum);
+        } else {
+            u64 v = valid ? sacc[w][lane] : 0;
+            addToRoots(rootCounts, valid, root, v);
+        }
         waveLdsSync();
     }
 }
-
 // ---------------------------------------------------------- TT-mode kernels
 // Leaf values per entry (instead of per-root atomics), then an upsweep that
 // sums each parent's contiguous child range, stores the subtree in the table,
@@ -446,8 +465,8 @@ __global__ __launch_bounds__(BS) void k_b_leaf1v(BatchDev *d)
 {
     if (d->oom) return;
     const u32 L = d->leafLevel;
-    const LvlDev &a = d->lv[L];
-    const LvlDev &p = d->lv[L - 1];
+    const LvlDev a = d->lv[L];
+    const LvlDev p = d->lv[L - 1];
     for (u64 e = (u64)blockIdx.x * BS + threadIdx.x; e < a.n; e += (u64)gridDim.x * BS) {
         u32 pi = a.idx[e];
         QuadBitBoard q = p.boards[pi];
@@ -466,8 +485,8 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
     __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
     __shared__ u32 sl[BL2_WAVES][BL2_CAP];
     const u32 L = d->leafLevel;
-    const LvlDev &a = d->lv[L];
-    const LvlDev &p = d->lv[L - 1];
+    const LvlDev a = d->lv[L];
+    const LvlDev p = d->lv[L - 1];
     const u64 n = a.n;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (;;) {
@@ -531,7 +550,7 @@ template <int C>  // side to move in level-L boards
 __global__ __launch_bounds__(BS) void k_b_upsweep(BatchDev *d, int L)
 {
     if (d->oom) return;
-    LvlDev &a = d->lv[L];
+    const LvlDev a = d->lv[L];
     const u64 *cv = d->lv[L + 1].val;
     const int rd = (int)d->depth - L;
     for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
@@ -554,7 +573,7 @@ __global__ __launch_bounds__(BS) void k_b_upsweep(BatchDev *d, int L)
 __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
 {
     if (d->oom) return;
-    const LvlDev &a = d->lv[0];
+    const LvlDev a = d->lv[0];
     const u64 *cv = d->lv[1].val;
     for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
         u64 v;
@@ -618,8 +637,10 @@ static int enqueueGroup(const GroupPlan &g, hipStream_t st)
             if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1v<WHITE>, dim3(G), dim3(BS), 0, st, d);
             else hipLaunchKernelGGL(k_b_leaf1v<BLACK>, dim3(G), dim3(BS), 0, st, d);
         } else {
-            if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2v<WHITE>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
-            else hipLaunchKernelGGL(k_b_leaf2v<BLACK>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+            unsigned R = residentGrid((const void *)(pside == WHITE ? k_b_leaf2v<WHITE> : k_b_leaf2v<BLACK>),
+                                      BL2_WAVES * 64);
+            if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2v<WHITE>, dim3(R), dim3(BL2_WAVES * 64), 0, st, d);
+            else hipLaunchKernelGGL(k_b_leaf2v<BLACK>, dim3(R), dim3(BL2_WAVES * 64), 0, st, d);
         }
         for (int L = g.leafLevel - 1; L >= 1; L--) {
             if ((L & 1) == WHITE) hipLaunchKernelGGL(k_b_upsweep<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
@@ -627,11 +648,14 @@ static int enqueueGroup(const GroupPlan &g, hipStream_t st)
         }
         hipLaunchKernelGGL(k_b_rootsum, dim3(G), dim3(BS), 0, st, d);
     } else if (g.plies == 1) {
-        if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1<WHITE>, dim3(G), dim3(BS), 0, st, d);
-        else hipLaunchKernelGGL(k_b_leaf1<BLACK>, dim3(G), dim3(BS), 0, st, d);
+        // work-queue leaves: exactly the resident blocks, so none starts late to an empty queue
+        unsigned R = residentGrid((const void *)(pside == WHITE ? k_b_leaf1<WHITE> : k_b_leaf1<BLACK>), BS);
+        if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1<WHITE>, dim3(R), dim3(BS), 0, st, d);
+        else hipLaunchKernelGGL(k_b_leaf1<BLACK>, dim3(R), dim3(BS), 0, st, d);
     } else {
-        if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2<WHITE>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
-        else hipLaunchKernelGGL(k_b_leaf2<BLACK>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+        unsigned R = residentGrid((const void *)(pside == WHITE ? k_b_leaf2<WHITE> : k_b_leaf2<BLACK>), BL2_WAVES * 64);
+        if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2<WHITE>, dim3(R), dim3(BL2_WAVES * 64), 0, st, d);
+        else hipLaunchKernelGGL(k_b_leaf2<BLACK>, dim3(R), dim3(BL2_WAVES * 64), 0, st, d);
     }
     PG_HIP(hipGetLastError());
     return PG_OK;
@@ -759,8 +783,11 @@ static int chooseLeafPlies(int depth, const std::vector<QuadBitBoard> &roots, co
     auto it = g_batch.tuned.find(key);
     if (it != g_batch.tuned.end()) return it->second;
     // time both leaf forms on a sample of the group (second run of each, so
-    // graph capture and first-launch costs are excluded)
-    u64 ns = std::min<u64>(roots.size(), 256);
+    // graph capture and first-launch costs are excluded); the sample shrinks
+    // 4x per ply above depth 4 so tuning a deep group stays cheap
+    u64 ns = 256;
+    for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
+    ns = std::min<u64>(roots.size(), ns);
     std::vector<u64> tmp(ns);
     double best = 1e30;
     int bestP = 2;

@@ -416,7 +416,9 @@ PG_HD void rayPair(BB enemy, BB own, BB empty, BB &att, BB &ownAtt)
 }
 
 // En-passant moves, reference semantics (MoveGeneratorBitboard.h:1732-1770,
-// 1639-1650).  Returns a mask of legal EP source pawns.
+// 1639-1650).  Returns a mask of legal EP source pawns.  Set-wise except for
+// the rare discovered rank check (the king on the capture rank with an enemy
+// rook or queen on it), which tests each source pawn.
 template <int C>
 PG_HD BB epSources(const Frame &f, u8 state)
 {
@@ -425,27 +427,26 @@ PG_HD BB epSources(const Frame &f, u8 state)
     const Sets &s = f.s;
     BB target = bit((ep - 1) + (C == WHITE ? 40 : 16));
     BB capSq = C == WHITE ? shift<-8>(target) : shift<8>(target);
-    BB myPawns = s.pawns & s.us;
-    BB src = ((shift<1>(capSq) & NOT_A) | (shift<-1>(capSq) & NOT_H)) & myPawns;
-    if (!src) return BB();
+    BB src = ((shift<1>(capSq) & NOT_A) | (shift<-1>(capSq) & NOT_H)) & s.pawns & s.us;
     if (f.nCheck) {
         // only when the captured pawn is the single checker, unpinned pawns only
         return (f.nCheck == 1 && capSq == f.checkers) ? (src & ~f.pinned) : BB();
     }
-    BB legal;
-    while (src) {
-        BB p = lowBit(src);
-        src ^= p;
-        if (p & f.pinned) {
-            // a pinned pawn may capture along its pin line only
-            BB line = (p & f.pinDiag) ? f.pinDiag : ((p & f.pinAnti) ? f.pinAnti : BB());
-            if (target & line) legal |= p;
-        } else {
-            // removing both pawns must not open the king's rank to a rook/queen
-            BB occ2 = s.occ ^ capSq ^ p;
-            BB r = lineAttacks(occ2, sqMasks(f.k).rank, f.kBit);
+    // a pinned pawn captures only along its own pin line: the target must lie
+    // on the pin rays of the diagonal (or anti-diagonal) the pawn is pinned on
+    BB onLine = ((target & f.pinDiag) ? f.pinDiag : BB()) | ((target & f.pinAnti) ? f.pinAnti : BB());
+    BB legal = src & f.pinned & onLine;
+    BB freeSrc = src & ~f.pinned;
+    // removing both pawns must not open the king's rank to a rook/queen
+    BB kRank = sqMasks(f.k).rank;
+    if ((kRank & capSq) && (kRank & s.orth & s.them)) {
+        for (BB b = freeSrc; b; b = clearLow(b)) {
+            BB p = lowBit(b);
+            BB r = lineAttacks(s.occ ^ capSq ^ p, kRank, f.kBit);
             if (!(r & s.orth & s.them)) legal |= p;
         }
+    } else {
+        legal |= freeSrc;
     }
     return legal;
 }

@@ -143,6 +143,10 @@ struct Lib {
 Lib &lib();
 
 // kernels (pg_kernels.hip)
+// Blocks of kernel `fn` (block size `blockSize`) resident on the whole device
+// at once: occupancy per CU x CUs.  A persistent grid of exactly this size
+// runs in one round; a larger one leaves a partly filled last round.
+unsigned residentGrid(const void *fn, int blockSize);
 // `pinned`: a host-pinned buffer of at least 8 u64 owned by the caller for
 // the call's readbacks (each concurrent caller brings its own)
 int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, bool useTT, void *arena,

@@ -17,6 +17,8 @@
 // PG_ERR_ARENA_OOM and the caller splits one ply down), sizes are 64-bit.
 #include <string.h>
 
+#include <unordered_map>
+
 #include "pg_internal.h"
 #include "pg_device.h"
 #include "pg_tt.h"
@@ -211,10 +213,13 @@ constexpr int L2_CHUNK = 64;
 constexpr int L2_CAP = 2048;
 constexpr int L2_WAVES = 4;
 
-template <int C, int CHUNK, int CAP>
+// Chunks are taken either by a static stride over the grid, or (QUEUE) from
+// an atomic counter `work` so that no wave idles while others still hold
+// several chunks (the counter starts at zero; the grid is the resident one).
+template <int C, int CHUNK, int CAP, bool QUEUE>
 __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
                                           const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
-                                          u64 *__restrict__ sum)
+                                          u64 *__restrict__ sum, u64 *work)
 {
     static_assert(CHUNK <= 64, "one entry per lane");
     __shared__ QuadBitBoard sb[L2_WAVES][CHUNK];
@@ -223,7 +228,18 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     u64 acc = 0;
     const u64 stride = (u64)gridDim.x * L2_WAVES * CHUNK;
-    for (u64 base = ((u64)blockIdx.x * L2_WAVES + w) * CHUNK; base < n; base += stride) {
+    u64 next = ((u64)blockIdx.x * L2_WAVES + w) * CHUNK;
+    for (;;) {
+        u64 base;
+        if constexpr (QUEUE) {
+            base = 0;
+            if (lane == 0) base = atomicAdd((unsigned long long *)work, (unsigned long long)CHUNK);
+            base = __shfl(base, 0);
+        } else {
+            base = next;
+            next += stride;
+        }
+        if (base >= n) break;
         u32 cnt = 0;
         u64 e = base + lane;
         if (lane < CHUNK && e < n) {
@@ -275,7 +291,15 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__r
                                                          const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
                                                          const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum)
 {
-    leaf2Body<C, L2_CHUNK, L2_CAP>(pb, ps, pidx, mv, n, sum);
+    leaf2Body<C, L2_CHUNK, L2_CAP, false>(pb, ps, pidx, mv, n, sum, nullptr);
+}
+template <int C>
+__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2q(const QuadBitBoard *__restrict__ pb,
+                                                          const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                          const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum,
+                                                          u64 *work)
+{
+    leaf2Body<C, L2_CHUNK, L2_CAP, true>(pb, ps, pidx, mv, n, sum, work);
 }
 template <int C>
 __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_c32(const QuadBitBoard *__restrict__ pb,
@@ -283,9 +307,10 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_c32(const QuadBitBoard 
                                                              const u16 *__restrict__ mv, u64 n,
                                                              u64 *__restrict__ sum)
 {
-    leaf2Body<C, 32, 1536>(pb, ps, pidx, mv, n, sum);
+    leaf2Body<C, 32, 1536, false>(pb, ps, pidx, mv, n, sum, nullptr);
 }
 int g_leaf2Variant = 0;
+int g_leafBlocksPerCU = 8;  // grid of the grid-stride leaves, per CU (x2 for the one-ply leaf)
 
 // ------------------------------------------------------------ TT-mode kernels
 // With a transposition table each expanded level probes the table at its
@@ -512,6 +537,20 @@ __global__ __launch_bounds__(BS) void k_sum_u64(const u64 *__restrict__ v, u64 n
 // ------------------------------------------------------------ launch helpers
 static inline unsigned gridFor(u64 n, int per) { return (unsigned)((n + per - 1) / per); }
 
+unsigned residentGrid(const void *fn, int blockSize)
+{
+    static std::mutex m;
+    static std::unordered_map<const void *, unsigned> cache;
+    std::lock_guard<std::mutex> g(m);
+    auto it = cache.find(fn);
+    if (it != cache.end()) return it->second;
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, blockSize, 0) != hipSuccess || perCU < 1) perCU = 1;
+    unsigned grid = (unsigned)perCU * (unsigned)(lib().cuCount ? lib().cuCount : 256);
+    cache.emplace(fn, grid);
+    return grid;
+}
+
 static int launchMakecount(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, const u16 *mv, int color, u64 n,
                            QuadBitBoard *ob, u8 *os, u32 *cnt, u32 *tileSums, hipStream_t st)
 {
@@ -559,25 +598,36 @@ static int launchGenmoves(const QuadBitBoard *boards, const u8 *states, const u3
     return PG_OK;
 }
 
+// `work`: a zeroed device u64 for the fused leaf's chunk queue, or null for
+// the static-stride form
 static int launchLeaf(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, const u16 *mv, int color, u64 n,
-                      int plies, u64 *sum, hipStream_t st)
+                      int plies, u64 *sum, hipStream_t st, u64 *work = nullptr)
 {
     if (!n) return PG_OK;
+    // grid-stride leaves: g_leafBlocksPerCU blocks per CU (several rounds of
+    // resident blocks; short per-block shares keep the last round's tail
+    // small), smaller when the work is smaller
+    const u64 cus = (u64)(lib().cuCount ? lib().cuCount : 256);
     if (plies == 1) {
-        u64 blocks = (n + BS - 1) / BS, cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 16;
+        u64 blocks = (n + BS - 1) / BS;
+        u64 cap = cus * 2 * g_leafBlocksPerCU;
         dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(BS);
         if (color == WHITE) hipLaunchKernelGGL(k_leaf1<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
         else hipLaunchKernelGGL(k_leaf1<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
     } else {
-        // persistent-style grid: enough waves to fill every CU, each wave strides over chunks
         const u64 chunk = g_leaf2Variant == 1 ? 32 : L2_CHUNK;
         u64 chunks = (n + chunk - 1) / chunk;
         u64 blocks = (chunks + L2_WAVES - 1) / L2_WAVES;
-        u64 cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 8;
+        u64 cap = cus * g_leafBlocksPerCU;
         dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(L2_WAVES * 64);
         if (g_leaf2Variant == 1) {
             if (color == WHITE) hipLaunchKernelGGL(k_leaf2_c32<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
             else hipLaunchKernelGGL(k_leaf2_c32<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        } else if (work) {
+            u64 r = residentGrid((const void *)(color == WHITE ? k_leaf2q<WHITE> : k_leaf2q<BLACK>), L2_WAVES * 64);
+            dim3 gq((unsigned)(blocks < r ? blocks : r));
+            if (color == WHITE) hipLaunchKernelGGL(k_leaf2q<WHITE>, gq, b, 0, st, pb, ps, pidx, mv, n, sum, work);
+            else hipLaunchKernelGGL(k_leaf2q<BLACK>, gq, b, 0, st, pb, ps, pidx, mv, n, sum, work);
         } else {
             if (color == WHITE) hipLaunchKernelGGL(k_leaf2<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
             else hipLaunchKernelGGL(k_leaf2<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
@@ -623,7 +673,7 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
     PG_HIP(hipMemcpyAsync(s0, &state, 1, hipMemcpyHostToDevice, st));
     PG_HIP(hipMemcpyAsync(mv, rootMoves, n1 * sizeof(u16), hipMemcpyHostToDevice, st));
     PG_HIP(hipMemsetAsync(idx, 0, n1 * sizeof(u32), st));
-    PG_HIP(hipMemsetAsync(dSum, 0, sizeof(u64), st));
+    PG_HIP(hipMemsetAsync(dSum, 0, 2 * sizeof(u64), st));
     PG_HIP(hipMemsetAsync(ctr, 0, 4 * sizeof(u64), st));
     std::vector<TTLevel> lv;
     const QuadBitBoard *pb = b0;
@@ -697,7 +747,7 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         } else {
             u64 chunks = (n + L2_CHUNK - 1) / L2_CHUNK;
             u64 blocks = (chunks + L2_WAVES - 1) / L2_WAVES;
-            u64 cap = (u64)(L.cuCount ? L.cuCount : 256) * 8;
+            u64 cap = (u64)(L.cuCount ? L.cuCount : 256) * g_leafBlocksPerCU;
             dim3 g((unsigned)(blocks < cap ? blocks : cap)), b(L2_WAVES * 64);
             if (pc == WHITE) hipLaunchKernelGGL(k_leaf2_val<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, leafVal, tt, mask, ctr);
             else hipLaunchKernelGGL(k_leaf2_val<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, leafVal, tt, mask, ctr);
@@ -763,7 +813,7 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
     }
     QuadBitBoard *b0 = A.alloc<QuadBitBoard>(1);
     u8 *s0 = A.alloc<u8>(1);
-    u64 *dSum = A.alloc<u64>(1);
+    u64 *dSum = A.alloc<u64>(2);  // [leaf sum, leaf chunk queue]
     u32 *idx = A.alloc<u32>(n1);
     u16 *mv = A.alloc<u16>(n1);
     if (!b0 || !s0 || !dSum || !idx || !mv) return PG_ERR_ARENA_OOM;
@@ -772,7 +822,7 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
     PG_HIP(hipMemcpyAsync(s0, &state, 1, hipMemcpyHostToDevice, st));
     PG_HIP(hipMemcpyAsync(mv, rootMoves, n1 * sizeof(u16), hipMemcpyHostToDevice, st));
     PG_HIP(hipMemsetAsync(idx, 0, n1 * sizeof(u32), st));
-    PG_HIP(hipMemsetAsync(dSum, 0, sizeof(u64), st));
+    PG_HIP(hipMemsetAsync(dSum, 0, 2 * sizeof(u64), st));
 
     const QuadBitBoard *pb = b0;
     const u8 *ps = s0;
@@ -826,7 +876,7 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
     int pc = color ^ ((last - 1) & 1);
     // leaf: 6K + 33P
     P.begin(leafPlies == 1 ? K_LEAF1 : K_LEAF2, st);
-    PG_TRY(launchLeaf(pb, ps, idx, mv, pc, n, leafPlies, dSum, st));
+    PG_TRY(launchLeaf(pb, ps, idx, mv, pc, n, leafPlies, dSum, st, dSum + 1));
     P.end(6.0 * n + 33.0 * nParents, (double)n, st);
     PG_HIP(hipMemcpyAsync(pinned, dSum, sizeof(u64), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
@@ -988,3 +1038,4 @@ extern "C" int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind)
 }
 
 extern "C" void pg_set_leaf2_variant(int v) { g_leaf2Variant = v == 1 ? 1 : 0; }
+extern "C" void pg_set_leaf_grid(int blocksPerCU) { g_leafBlocksPerCU = blocksPerCU >= 1 && blocksPerCU <= 256 ? blocksPerCU : 8; }
