@@ -1,7 +1,7 @@
 """Benchmark: perft nodes/s on one MI355X.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--leaf 1|2]
-                  [--batch] [--serve N] [--serve-rate R]
+                  [--no-batch] [--no-tt] [--serve N] [--serve-rate R]
 
 Workload (one step): perft(7) of the config-2 position -- the start position
 after 8 seeded random plies (seed 2, SplitMix64 walk over canonically sorted
@@ -21,11 +21,13 @@ The JSON line carries:
   cpu_baseline  the oracle (a C restatement of the reference's single-threaded
                 -cpu path) on this host, one core, on a bounded sample of the
                 same tree: perft(6) under the first root moves.
-With --batch it also times the 4096-position config-3 batch, batched vs plain
-(both checked against the golden counts).  With --serve N it replays the first
-N config-5 requests through the serving loop with Poisson arrivals (rate R
-requests/s; default 0.75 x the capacity measured by a closed-loop pass) and
-reports p50/p99 latency and nodes/s.
+After the timed config-2 steps (the `value`) it runs three more legs, each
+checked against golden counts: the 4096-position config-3 batch, batched vs
+plain; config 4 (64 positions, perft 8) with the transposition tables,
+reporting table probes/s; and the config-5 serving loop, replaying N requests
+(default all 10,000) with Poisson arrivals (rate R requests/s; default 0.75 x
+the capacity measured by a closed-loop pass), reporting p50/p99 latency and
+nodes/s.
 """
 from __future__ import annotations
 
@@ -67,9 +69,10 @@ def cpu_baseline(fen: str, depth: int, budget_s: float = 15.0) -> dict:
 # HBM bytes per launch of the default config-2 leaf, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of this bench, each scaled by
 # what the counter reads for a plain 256 MiB-in / 256 MiB-out elementwise
-# kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0).
+# kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0);
+# the writes are mostly the chunk queue's atomics.
 # Commands and figures: profiles/leaf2_config2_pmc.md.
-PMC_TRAFFIC = {"leaf2": 61639.1875 * 1024 * 2.0 + 256.0 * 1024 * 1.0}
+PMC_TRAFFIC = {"leaf2": 66232.0 * 1024 * 2.0 + 7586.3 * 1024 * 1.0}
 
 
 def roofline(stat: dict, traffic=None) -> dict:
@@ -102,6 +105,39 @@ def batch_leg(pg) -> dict:
     return {"workload": "config-3: 4096 positions, 6-40 random plies, depth 3-6, no TT", "nodes": total,
             "batched_s": t_b, "batched_nodes_per_s": total / t_b, "plain_s": t_plain,
             "plain_nodes_per_s": total / t_plain, "batched_over_plain": t_plain / t_b}
+
+
+def tt_leg(pg) -> dict:
+    """Config 4: 64 positions (10 random plies), perft 8 with the transposition
+    tables (64 GB device table + 32 GB host tier), through the launcher (host
+    recursion above the launch depth, frontier batched on the device)."""
+    rows = load_fixture("c4.jsonl")
+    ps = [pg.parse_fen(r["fen"]) for r in rows]
+    pg.tt_init(65536, 32768)
+    # exactness: depth-5 golden counts (oracle), then TT on == TT off at depth 7
+    for r, p in zip(rows, ps):
+        assert pg.perft(p, r["golden_depth"], use_tt=True) == r["golden_count"]
+    d7_tt = [pg.perft(p, 7, use_tt=True) for p in ps]
+    d7 = [pg.perft(p, 7) for p in ps]
+    assert d7_tt == d7
+    pg.lib().pg_tt_clear()
+    stats0 = pg.tt_stats()
+    t0 = time.perf_counter()
+    counts = [pg.perft(p, 8, use_tt=True) for p in ps]
+    dt = time.perf_counter() - t0
+    stats = [b - a for a, b in zip(stats0, pg.tt_stats())]
+    # TT off on the first 4 positions at full depth as a spot check
+    spot = [pg.perft(p, 8) for p in ps[:4]]
+    assert spot == counts[:4], (spot, counts[:4])
+    pg.tt_free()
+    nodes = sum(counts)
+    return {"workload": "config-4: 64 positions (10 plies, seeds 100-163), perft 8, device TT 64 GB + host TT 32 GB, "
+                        "launcher (auto launch depth), one position after another, tables shared",
+            "nodes": nodes, "seconds": dt, "nodes_per_s": nodes / dt,
+            "device_tt_probes": stats[0], "device_tt_hits": stats[1], "device_tt_stores": stats[2],
+            "host_tt_probes": stats[3], "host_tt_hits": stats[4], "host_tt_stores": stats[5],
+            "tt_probes_per_s": (stats[0] + stats[3]) / dt,
+            "checked": "depth-5 oracle counts; TT on == TT off at depth 7 (all 64) and depth 8 (first 4)"}
 
 
 def serve_leg(pg, n_req: int, rate: float) -> dict:
@@ -172,6 +208,21 @@ def serve_leg(pg, n_req: int, rate: float) -> dict:
             "golden_checked": sum(1 for k in done if k in gold), "mismatches": len(bad)}
 
 
+def copy_probe(pg) -> None:
+    """Calibration kernel of known size for HBM counters and the achievable
+    copy rate: three device-to-device copies of 256 MiB (the runtime's copy
+    kernel reads 256 MiB and writes 256 MiB each)."""
+    n = 256 << 20
+    a, b = pg.DeviceArray(n), pg.DeviceArray(n)
+    a.zero()
+    for _ in range(3):
+        rc = pg.lib().pg_memcpy(b.ptr, a.ptr, n, 2)
+        if rc:
+            raise RuntimeError(pg.last_error())
+    pg.device_synchronize()
+    print(f"copy probe: 3 device copies of {n} bytes (read {n} + write {n} each)", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,9 +232,11 @@ def main():
     ap.add_argument("--leaf2-variant", type=int, default=0)
     ap.add_argument("--leaf-grid", type=int, default=0, help="leaf blocks per CU (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", action="store_true")
-    ap.add_argument("--serve", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched-vs-plain leg")
+    ap.add_argument("--no-tt", action="store_true", help="skip the config-4 transposition-table leg")
+    ap.add_argument("--serve", type=int, default=10000, help="config-5 requests to replay (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0)
+    ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -193,6 +246,10 @@ def main():
 
     import perft_gpu_2026_amd as pg
 
+    if a.copy_probe:
+        pg.init(arena_bytes=1 << 30)
+        copy_probe(pg)
+        return
     fx = load_fixture("c2.jsonl")[0]
     depth, want = fx["depth"], fx["counts"][-1]
     print(f"[bench] init; config-2 position {fx['fen']} depth {depth}", file=sys.stderr, flush=True)
@@ -247,9 +304,12 @@ def main():
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
                     for k, v in stats.items()},
     }
-    if a.batch:
+    if not a.no_batch:
         print("[bench] config-3 batch", file=sys.stderr, flush=True)
         out["batch_config3"] = batch_leg(pg)
+    if not a.no_tt:
+        print("[bench] config-4 transposition tables", file=sys.stderr, flush=True)
+        out["tt_config4"] = tt_leg(pg)
     if a.serve:
         print("[bench] config-5 serving", file=sys.stderr, flush=True)
         out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate)

@@ -160,50 +160,21 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
     }
 }
 
-__global__ __launch_bounds__(1024) void k_b_scan_top(BatchDev *d, int L)
+__global__ __launch_bounds__(TOP_BS) void k_b_scan_top(BatchDev *d, int L)
 {
     if (d->oom) return;
     const LvlDev a = d->lv[L];
-    u64 tiles = (a.n + BS - 1) / BS;
-    __shared__ u64 wtot[16];
-    __shared__ u64 carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (u64 base = 0; base < tiles; base += 1024) {
-        u64 i = base + threadIdx.x;
-        u64 v = i < tiles ? a.tileSums[i] : 0;
-        u64 incl = waveInclScan64(v);
-        if (lane == 63) wtot[w] = incl;
-        __syncthreads();
-        u64 before = carry;
-        for (int k = 0; k < w; k++) before += wtot[k];
-        if (i < tiles) a.tileOffs[i] = before + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = before + incl;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.tileOffs[tiles] = carry;
+    scanTileSums(a.tileSums, (a.n + BS - 1) / BS, a.tileOffs);
 }
 
 __global__ __launch_bounds__(BS) void k_b_scan_apply(BatchDev *d, int L)
 {
     if (d->oom) return;
     const LvlDev a = d->lv[L];
-    u64 tiles = (a.n + BS - 1) / BS;
-    __shared__ u32 wtot[BS / 64];
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
-        u64 i = t * BS + threadIdx.x;
-        u32 v = i < a.n ? a.cnt[i] : 0;
-        u32 incl = waveInclScan(v);
-        if (lane == 63) wtot[w] = incl;
-        __syncthreads();
-        u32 before = 0;
-        for (int k = 0; k < w; k++) before += wtot[k];
-        if (i < a.n) a.cnt[i] = (u32)(a.tileOffs[t] + before + incl);
-        __syncthreads();
-    }
+    const u64 tiles = (a.n + BS - 1) / BS;
+    // one wave per 256-entry tile (no block barrier inside)
+    for (u64 t = (u64)blockIdx.x * (BS / 64) + (threadIdx.x >> 6); t < tiles; t += (u64)gridDim.x * (BS / 64))
+        scanApplyTile(a.cnt, a.n, a.tileOffs, t);
 }
 
 // one thread: size the next level and carve its arrays out of the arena
@@ -251,38 +222,63 @@ __global__ __launch_bounds__(BS) void k_b_partitions(BatchDev *d, int L)
     }
 }
 
+// parent index (and root id) of every child of level L+1: one 2048-child tile
+// per block iteration, parents' first children scattered into an LDS tile and
+// filled forward by a max-scan (the scheme of k_expand)
 __global__ __launch_bounds__(BS) void k_b_expand(BatchDev *d, int L)
 {
     if (d->oom) return;
     const LvlDev a = d->lv[L];
     const LvlDev b = d->lv[L + 1];
-    u64 c = b.n;
-    u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
-    __shared__ u32 win[EXPAND_WCAP];
+    constexpr int PER = EXPAND_TILE / BS;
+    __shared__ u32 tile[EXPAND_TILE];
+    __shared__ u32 wmax[BS / 64];
+    const u64 c = b.n;
+    const u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
     const u32 *incl = a.cnt;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
-        u64 c0 = t * EXPAND_TILE;
-        u64 p0 = a.parts[t], p1 = a.parts[t + 1];
-        u64 w = p1 - p0 + 1;
-        bool staged = w <= EXPAND_WCAP;
-        if (staged)
-            for (u64 k = threadIdx.x; k < w; k += BS) win[k] = incl[p0 + k];
+        const u64 c0 = t * EXPAND_TILE;
+        const u64 p0 = a.parts[t], p1 = a.parts[t + 1];
+        for (int k = tid; k < EXPAND_TILE; k += BS) tile[k] = 0;
         __syncthreads();
-        for (int k = 0; k < EXPAND_TILE / BS; k++) {
-            u64 ch = c0 + (u64)k * BS + threadIdx.x;
-            if (ch >= c) break;
-            u64 lo = 0, hi = w;
-            while (lo < hi) {
-                u64 mid = (lo + hi) >> 1;
-                u32 v = staged ? win[mid] : incl[p0 + mid];
-                if ((u64)v <= ch) lo = mid + 1;
-                else hi = mid;
+        for (u64 p = p0 + tid; p <= p1; p += BS) {
+            u64 ex = p ? incl[p - 1] : 0, in = incl[p];
+            if (in > ex) {
+                u64 st = ex > c0 ? ex - c0 : 0;
+                if (st < EXPAND_TILE) tile[st] = (u32)p + 1;
             }
-            u32 parent = (u32)(p0 + lo);
-            b.idx[ch] = parent;
-            b.root[ch] = L == 0 ? parent : a.root[parent];
         }
         __syncthreads();
+        u32 v[PER];
+        u32 m = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            u32 x = tile[tid * PER + k];
+            m = x > m ? x : m;
+            v[k] = m;
+        }
+        u32 wm = m;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            u32 tt = __shfl_up(wm, o);
+            if (lane >= o) wm = tt > wm ? tt : wm;
+        }
+        if (lane == 63) wmax[w] = wm;
+        u32 prev = __shfl_up(wm, 1);
+        if (lane == 0) prev = 0;
+        __syncthreads();
+        for (int k = 0; k < w; k++) prev = wmax[k] > prev ? wmax[k] : prev;
+        const u64 base = c0 + (u64)tid * PER;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            if (base + k < c) {
+                u32 parent = (v[k] > prev ? v[k] : prev) - 1;
+                b.idx[base + k] = parent;
+                b.root[base + k] = L == 0 ? parent : a.root[parent];
+            }
+        }
+        __syncthreads();  // the tile is cleared again by the next iteration
     }
 }
 
@@ -623,7 +619,7 @@ static int enqueueGroup(const GroupPlan &g, hipStream_t st)
             if (side == BLACK) hipLaunchKernelGGL(k_b_makecount<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
             else hipLaunchKernelGGL(k_b_makecount<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
         }
-        hipLaunchKernelGGL(k_b_scan_top, dim3(1), dim3(1024), 0, st, d, L);
+        hipLaunchKernelGGL(k_b_scan_top, dim3(1), dim3(TOP_BS), 0, st, d, L);
         hipLaunchKernelGGL(k_b_scan_apply, dim3(G), dim3(BS), 0, st, d, L);
         hipLaunchKernelGGL(k_b_alloc, dim3(1), dim3(1), 0, st, d, L);
         hipLaunchKernelGGL(k_b_partitions, dim3(G), dim3(BS), 0, st, d, L);

@@ -6,7 +6,6 @@ namespace pg {
 
 constexpr int BS = 256;          // threads per block: 4 wave64
 constexpr int EXPAND_TILE = 2048;  // children per expand block (8 per thread)
-constexpr int EXPAND_WCAP = 2048;  // parents staged in LDS per expand block
 constexpr int GEN_LCAP = 16384;    // moves staged in LDS per genmoves block (32 KB)
 
 // ------------------------------------------------------------ wave64 helpers
@@ -54,6 +53,66 @@ __device__ __forceinline__ u64 blockSum64(u64 v)
     if (threadIdx.x == 0)
         for (int w = 0; w < BS / 64; w++) t += part[w];
     return t;
+}
+
+// Exclusive scan of `tiles` tile sums into tileOffs[0..tiles], tileOffs[tiles]
+// = total, by one block of TOP_BS threads: each thread sums a run of
+// consecutive tiles, the block scans the run totals, and each thread writes
+// its run's offsets (one barrier, whatever the number of tiles).
+constexpr int TOP_BS = 1024;
+__device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u64 tiles, u64 *__restrict__ tileOffs)
+{
+    __shared__ u64 wtot[TOP_BS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const u64 run = (tiles + TOP_BS - 1) / TOP_BS;
+    const u64 b0 = (u64)threadIdx.x * run;
+    const u64 b1 = b0 + run < tiles ? b0 + run : tiles;
+    u64 local = 0;
+#pragma unroll 8
+    for (u64 i = b0; i < b1; i++) local += tileSums[i];
+    u64 incl = waveInclScan64(local);
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    u64 acc = incl - local;
+    for (int k = 0; k < w; k++) acc += wtot[k];
+#pragma unroll 8
+    for (u64 i = b0; i < b1; i++) {
+        tileOffs[i] = acc;
+        acc += tileSums[i];
+    }
+    if (threadIdx.x == TOP_BS - 1) tileOffs[tiles] = acc;
+}
+
+// One wave turns the 256 counts of `tile` into inclusive counts: lane l holds
+// entries 4l..4l+3 of the tile (a 16-byte access when whole and aligned),
+// scans them locally, and adds the wave's exclusive lane prefix and the
+// tile's offset.  Waves whose tile starts past n do nothing.
+__device__ __forceinline__ void scanApplyTile(u32 *__restrict__ d, u64 n, const u64 *__restrict__ tileOffs, u64 tile)
+{
+    const int lane = threadIdx.x & 63;
+    if (tile * BS >= n) return;
+    const u64 i0 = tile * BS + (u64)lane * 4;
+    const bool vec = i0 + 4 <= n && ((uintptr_t)d & 15) == 0;
+    u32 x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    if (vec) {
+        uint4 q = *(const uint4 *)(d + i0);
+        x0 = q.x, x1 = q.y, x2 = q.z, x3 = q.w;
+    } else {
+        if (i0 < n) x0 = d[i0];
+        if (i0 + 1 < n) x1 = d[i0 + 1];
+        if (i0 + 2 < n) x2 = d[i0 + 2];
+        if (i0 + 3 < n) x3 = d[i0 + 3];
+    }
+    u32 s1 = x0 + x1, s2 = s1 + x2, s3 = s2 + x3;
+    u32 base = (u32)tileOffs[tile] + waveInclScan(s3) - s3;
+    if (vec) {
+        *(uint4 *)(d + i0) = make_uint4(base + x0, base + s1, base + s2, base + s3);
+    } else {
+        if (i0 < n) d[i0] = base + x0;
+        if (i0 + 1 < n) d[i0 + 1] = base + s1;
+        if (i0 + 2 < n) d[i0 + 2] = base + s2;
+        if (i0 + 3 < n) d[i0 + 3] = base + s3;
+    }
 }
 
 // first p in [0, n) with incl[p] > c (n if none)

@@ -3,11 +3,12 @@
 //
 // One BFS level (entries = (parent index, move) pairs):
 //   k_makecount   child board = make(parent board, move); count its moves;
-//                 per-256-entry tile sums for the scan            (HBM bound)
+//                 per-256-entry tile sums for the scan            (VALU bound)
 //   k_scan_top    exclusive scan of the tile sums (one block)
-//   k_scan_apply  wave64 block scan + tile offset -> inclusive counts
+//   k_scan_apply  wave64 block scan + tile offset -> inclusive counts (HBM)
 //   k_partitions  one upper_bound per 2048-child tile
-//   k_expand      parent index of every child, window staged in LDS
+//   k_expand      parent index of every child: first-child scatter + max-scan
+//                 in LDS, coalesced 16-byte stores                 (HBM)
 //   k_genmoves    moves of every board, staged in LDS, written coalesced
 // and after the last level a leaf kernel sums perft(1) or perft(2) of every
 // entry (k_leaf; the 2-ply form is the reference's fused leaf, here with the
@@ -56,44 +57,19 @@ __global__ __launch_bounds__(BS) void k_tile_sums(const u32 *__restrict__ d, u64
     if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)s;
 }
 
-// exclusive scan of `tiles` tile sums into tileOffs[0..tiles], tileOffs[tiles] = total
-constexpr int TOP_BS = 1024;
+// exclusive scan of `tiles` tile sums into tileOffs[0..tiles] (one block)
 __global__ __launch_bounds__(TOP_BS) void k_scan_top(const u32 *__restrict__ tileSums, u64 tiles,
                                                      u64 *__restrict__ tileOffs)
 {
-    __shared__ u64 wtot[TOP_BS / 64];
-    __shared__ u64 carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (u64 base = 0; base < tiles; base += TOP_BS) {
-        u64 i = base + threadIdx.x;
-        u64 v = i < tiles ? tileSums[i] : 0;
-        u64 incl = waveInclScan64(v);
-        if (lane == 63) wtot[w] = incl;
-        __syncthreads();
-        u64 before = carry;
-        for (int k = 0; k < w; k++) before += wtot[k];
-        if (i < tiles) tileOffs[i] = before + incl - v;
-        __syncthreads();
-        if (threadIdx.x == TOP_BS - 1) carry = before + incl;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) tileOffs[tiles] = carry;
+    scanTileSums(tileSums, tiles, tileOffs);
 }
 
+// Inclusive counts from the tile offsets: one wave per 256-entry tile, four
+// consecutive entries per lane (one 16-byte load and store), a lane-local
+// scan of the four and one wave scan of the lane totals; no block barrier.
 __global__ __launch_bounds__(BS) void k_scan_apply(u32 *__restrict__ d, u64 n, const u64 *__restrict__ tileOffs)
 {
-    __shared__ u32 wtot[BS / 64];
-    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
-    u32 v = i < n ? d[i] : 0;
-    u32 incl = waveInclScan(v);
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 63) wtot[w] = incl;
-    __syncthreads();
-    u32 before = 0;
-    for (int k = 0; k < w; k++) before += wtot[k];
-    if (i < n) d[i] = (u32)(tileOffs[blockIdx.x] + before + incl);
+    scanApplyTile(d, n, tileOffs, (u64)blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
 }
 
 // parts[t] = parent of child t*EXPAND_TILE, for t in [0, tiles]
@@ -108,37 +84,63 @@ __global__ __launch_bounds__(BS) void k_partitions(const u32 *__restrict__ incl,
     parts[t] = (u32)(p < n ? p : n - 1);
 }
 
+// Parent index of every child of one 2048-child tile.  Each parent whose
+// first child falls in the tile writes its index (+1) at that position of an
+// LDS tile (zero elsewhere; the parent holding the tile's first child writes
+// position 0); an inclusive max-scan then fills every child with its parent.
+// Per child: one LDS write (the clear), one read, a max, one coalesced 4-byte
+// store -- no per-child search.
 __global__ __launch_bounds__(BS) void k_expand(const u32 *__restrict__ incl, u64 n, u64 c,
                                                const u32 *__restrict__ parts, u32 *__restrict__ parentIdx)
 {
-    __shared__ u32 win[EXPAND_WCAP];
-    u64 c0 = (u64)blockIdx.x * EXPAND_TILE;
-    u64 p0 = parts[blockIdx.x], p1 = parts[blockIdx.x + 1];
-    u64 w = p1 - p0 + 1;
-    bool staged = w <= EXPAND_WCAP;
-    if (staged) {
-        for (u64 k = threadIdx.x; k < w; k += BS) win[k] = incl[p0 + k];
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < EXPAND_TILE / BS; k++) {
-        u64 ch = c0 + (u64)k * BS + threadIdx.x;
-        if (ch >= c) break;
-        u64 lo = 0, hi = w;
-        if (staged) {
-            while (lo < hi) {
-                u64 mid = (lo + hi) >> 1;
-                if ((u64)win[mid] <= ch) lo = mid + 1;
-                else hi = mid;
-            }
-        } else {
-            while (lo < hi) {
-                u64 mid = (lo + hi) >> 1;
-                if ((u64)incl[p0 + mid] <= ch) lo = mid + 1;
-                else hi = mid;
-            }
+    constexpr int PER = EXPAND_TILE / BS;  // 8 consecutive children per thread
+    __shared__ u32 tile[EXPAND_TILE];
+    __shared__ u32 wmax[BS / 64];
+    const u64 c0 = (u64)blockIdx.x * EXPAND_TILE;
+    const u64 p0 = parts[blockIdx.x], p1 = parts[blockIdx.x + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int k = tid; k < EXPAND_TILE; k += BS) tile[k] = 0;
+    __syncthreads();
+    for (u64 p = p0 + tid; p <= p1; p += BS) {
+        u64 ex = p ? incl[p - 1] : 0, in = incl[p];
+        if (in > ex) {
+            u64 st = ex > c0 ? ex - c0 : 0;
+            if (st < EXPAND_TILE) tile[st] = (u32)p + 1;
         }
-        parentIdx[ch] = (u32)(p0 + lo);
+    }
+    __syncthreads();
+    u32 v[PER];
+    u32 m = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        u32 x = tile[tid * PER + k];
+        m = x > m ? x : m;
+        v[k] = m;
+    }
+    // running max across the wave, then across the block's waves
+    u32 wm = m;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        u32 t = __shfl_up(wm, o);
+        if (lane >= o) wm = t > wm ? t : wm;
+    }
+    if (lane == 63) wmax[w] = wm;
+    u32 prev = __shfl_up(wm, 1);
+    if (lane == 0) prev = 0;
+    __syncthreads();
+    for (int k = 0; k < w; k++) prev = wmax[k] > prev ? wmax[k] : prev;
+    const u64 base = c0 + (u64)tid * PER;
+    u32 o[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) o[k] = (v[k] > prev ? v[k] : prev) - 1;
+    if (base + PER <= c && ((uintptr_t)parentIdx & 15) == 0) {
+        uint4 *dst = (uint4 *)(parentIdx + base);
+#pragma unroll
+        for (int k = 0; k < PER / 4; k++) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (base + k < c) parentIdx[base + k] = o[k];
     }
 }
 
@@ -571,7 +573,7 @@ static int launchScan(u32 *d, u64 n, const u32 *tileSums, u64 *tileOffs, hipStre
     if (!n) return PG_OK;
     u64 tiles = (n + BS - 1) / BS;
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_BS), 0, st, tileSums, tiles, tileOffs);
-    hipLaunchKernelGGL(k_scan_apply, dim3(gridFor(n, BS)), dim3(BS), 0, st, d, n, tileOffs);
+    hipLaunchKernelGGL(k_scan_apply, dim3(gridFor(n, 4 * BS)), dim3(BS), 0, st, d, n, tileOffs);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
