@@ -24,7 +24,9 @@ The JSON line carries:
 After the timed config-2 steps (the `value`) it runs three more legs, each
 checked against golden counts: the 4096-position config-3 batch, batched vs
 plain; config 4 (64 positions, perft 8) with the transposition tables,
-reporting table probes/s; and the config-5 serving loop, replaying N requests
+reporting table probes/s; the reference README's published positions
+(startpos perft 9, Kiwipete perft 7, Position 3 perft 10, no TT) with the
+published times beside them; and the config-5 serving loop, replaying N requests
 (default all 10,000) with Poisson arrivals (rate R requests/s; default 0.75 x
 the capacity measured by a closed-loop pass), reporting p50/p99 latency and
 nodes/s.
@@ -140,6 +142,30 @@ def tt_leg(pg) -> dict:
             "checked": "depth-5 oracle counts; TT on == TT off at depth 7 (all 64) and depth 8 (first 4)"}
 
 
+# The reference's own published single-position runs (no TT), for context:
+# README.md:51-52 (RTX PRO 6000 Blackwell) and :74 (RTX 4090).
+REF_RUNS = [("startpos", 9, "1.867 s on RTX PRO 6000"), ("kiwipete", 7, "0.177 s on RTX PRO 6000"),
+            ("position3", 10, "1.55 s on RTX 4090")]
+
+
+def ref_positions_leg(pg) -> dict:
+    """The reference README's published positions through the launcher (no TT),
+    each count checked against the published value."""
+    known = {p["name"]: p for p in json.load(open(os.path.join(ROOT, "tests", "golden", "known_positions.json")))["positions"]}
+    out = {}
+    for name, d, theirs in REF_RUNS:
+        k = known[name]
+        want = k["counts"][d - 1]
+        p = pg.parse_fen(k["fen"])
+        pg.perft(p, d - 2)  # warm (launch depth estimate, first launches)
+        t0 = time.perf_counter()
+        c = pg.perft(p, d)
+        dt = time.perf_counter() - t0
+        assert c == want, (name, d, c, want)
+        out[f"{name}_perft{d}"] = {"nodes": c, "seconds": dt, "nodes_per_s": c / dt, "reference_published": theirs}
+    return out
+
+
 def serve_leg(pg, n_req: int, rate: float) -> dict:
     import ctypes
     import random
@@ -234,6 +260,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched-vs-plain leg")
     ap.add_argument("--no-tt", action="store_true", help="skip the config-4 transposition-table leg")
+    ap.add_argument("--no-ref-positions", action="store_true", help="skip the reference README positions")
     ap.add_argument("--serve", type=int, default=10000, help="config-5 requests to replay (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0)
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
@@ -310,6 +337,9 @@ def main():
     if not a.no_tt:
         print("[bench] config-4 transposition tables", file=sys.stderr, flush=True)
         out["tt_config4"] = tt_leg(pg)
+    if not a.no_ref_positions:
+        print("[bench] reference README positions", file=sys.stderr, flush=True)
+        out["reference_positions"] = ref_positions_leg(pg)
     if a.serve:
         print("[bench] config-5 serving", file=sys.stderr, flush=True)
         out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate)

@@ -38,6 +38,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_gen_moves.restype = c_uint32
         L.oracle_brute_gen_moves.argtypes = [u64x4, c_uint8, c_uint8, POINTER(c_uint16)]
         L.oracle_brute_gen_moves.restype = c_uint32
+        L.oracle_brute_count_moves.argtypes = [u64x4, c_uint8, c_uint8]
+        L.oracle_brute_count_moves.restype = c_uint32
         L.oracle_make_move.argtypes = [u64x4, POINTER(c_uint8), c_uint8, c_uint16]
         L.oracle_make_move.restype = None
         for f in ("oracle_perft", "oracle_perft_brute"):
@@ -101,6 +103,10 @@ def brute_gen_moves(p: Pos) -> list[int]:
     mv = (c_uint16 * 512)()
     n = lib().oracle_brute_gen_moves(p.arr(), p.state, p.color, mv)
     return list(mv[:n])
+
+
+def brute_count_moves(p: Pos) -> int:
+    return lib().oracle_brute_count_moves(p.arr(), p.state, p.color)
 
 
 def make_move(p: Pos, m: int) -> Pos:

@@ -200,6 +200,13 @@ static inline void emit_pawn(Sink *s, int from, int to, int isCap)
  * check by rook/queen only :1756-1767, EP evasion only when the captured pawn
  * is the checker :1639-1650) and castling on rights + empty path + unattacked
  * path without a rook-presence test (:1896-1913).
+ *
+ * One difference between counting and generating is the reference's own: out
+ * of check, countMovesOutOfCheck ORs the two en-passant capture directions
+ * onto the single target square and popcounts that (:1646-1648), so it counts
+ * at most ONE en-passant evasion when two pawns can take, while
+ * generateMovesOutOfCheck emits one per pawn (:1085-1090).  perft(1) is that
+ * count (launcher.cu:765-766, 809-810), so the count mode here does the same.
  */
 static u32 legal_moves(const u64 bb[4], u8 state, int color, Sink *s)
 {
@@ -213,7 +220,6 @@ static u32 legal_moves(const u64 bb[4], u8 state, int color, Sink *s)
     u64 pinned = pinned_pieces(&p, k, enemy);
     u64 threatened = attacked_by(&p, !color, myKing);
     u64 myPawns = p.pawns & mine;
-    u64 promoRanks = RANK_1 | RANK_8;
     u64 dpRank = color == WHITE ? RANK_3 : RANK_6;
 
     /* king moves (both paths) */
@@ -237,6 +243,7 @@ static u32 legal_moves(const u64 bb[4], u8 state, int color, Sink *s)
         if (attackers & (attackers - 1)) return s->n;   /* double check: king only */
         u64 safe = attackers | BETWEEN[k][lsb(attackers)];
         u64 movers = mine & ~pinned;
+        int epCounted = 0;
         u64 pw = myPawns & ~pinned;
         for (u64 b = pw; b; b &= b - 1) {
             int from = lsb(b);
@@ -251,7 +258,8 @@ static u32 legal_moves(const u64 bb[4], u8 state, int color, Sink *s)
             if (epTarget && epCaptured == attackers) {
                 u64 epc = (color == WHITE ? (east(north(pb)) | west(north(pb))) : (east(south(pb)) | west(south(pb))))
                           & epTarget;
-                if (epc) emit(s, from, lsb(epc), F_EP);
+                if (epc && (s->out || !epCounted)) emit(s, from, lsb(epc), F_EP);
+                if (epc) epCounted = 1;
             }
         }
         for (u64 b = p.knights & movers; b; b &= b - 1) emit_targets(s, lsb(b), KNIGHT_ATT[lsb(b)] & safe, enemy);
@@ -582,13 +590,26 @@ uint32_t oracle_brute_gen_moves(const uint64_t bb[4], uint8_t state, uint8_t col
     return n;
 }
 
+/* The reference's perft(1) convention on top of the brute generator: every
+ * legal move counts, except that a side in check counts its en-passant
+ * captures as one move however many pawns can take (countMovesOutOfCheck,
+ * MoveGeneratorBitboard.h:1639-1649). */
+uint32_t oracle_brute_count_moves(const uint64_t bb[4], uint8_t state, uint8_t color)
+{
+    u16 moves[256];
+    u32 n = oracle_brute_gen_moves(bb, state, color, moves), nEp = 0;
+    for (u32 i = 0; i < n; i++) nEp += (moves[i] >> 12) == F_EP;
+    if (nEp > 1 && king_attacked(bb, color)) n -= nEp - 1;
+    return n;
+}
+
 uint64_t oracle_perft_brute(const uint64_t bb[4], uint8_t state, uint8_t color, uint32_t depth)
 {
     oracle_init();
     if (depth == 0) return 1;
+    if (depth == 1) return oracle_brute_count_moves(bb, state, color);
     u16 moves[256];
     u32 n = oracle_brute_gen_moves(bb, state, color, moves);
-    if (depth == 1) return n;
     uint64_t total = 0;
     for (u32 i = 0; i < n; i++) {
         u64 c[4] = { bb[0], bb[1], bb[2], bb[3] };

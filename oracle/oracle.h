@@ -19,9 +19,16 @@
  * Pinning (what fixes the results, see tests/test_oracle.py):
  *   - published perft counts of six standard positions (start, Kiwipete,
  *     positions 3-6) and the reference README's own counts (README.md:51-103);
- *   - count == |generate| on every node of full trees;
+ *   - count == |generate| on every node of full trees, except where the
+ *     reference's own count differs from its generator (below);
  *   - fast path == brute-force path on seeded random positions;
  *   - the 218-move position of chess.h:177.
+ *
+ * Counting (perft(1)) follows the reference's countMovesOutOfCheck, which
+ * counts the en-passant evasions of a side in check as one move even when two
+ * pawns can take (MoveGeneratorBitboard.h:1639-1649; its generator emits both,
+ * :1085-1090).  Position 3 at depth 10 shows it: 860,322,381,070 (the
+ * reference's README.md count) against 860,322,602,309 legal leaves.
  */
 #ifndef PG_ORACLE_H
 #define PG_ORACLE_H
@@ -51,6 +58,9 @@ void     oracle_perft_batch(int n, const uint64_t *bbs, const uint8_t *states, c
 
 /* brute force: pseudo-legal moves filtered by "own king not attacked after make" */
 uint32_t oracle_brute_gen_moves(const uint64_t bb[4], uint8_t state, uint8_t color, uint16_t *moves);
+/* the reference's perft(1) count from the brute generator: in check, all
+ * en-passant captures together count once (MoveGeneratorBitboard.h:1639-1649) */
+uint32_t oracle_brute_count_moves(const uint64_t bb[4], uint8_t state, uint8_t color);
 uint64_t oracle_perft_brute(const uint64_t bb[4], uint8_t state, uint8_t color, uint32_t depth);
 
 /* seeded random walk: SplitMix64(seed), legal moves sorted by (from,to,promo),

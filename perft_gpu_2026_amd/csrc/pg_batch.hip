@@ -118,7 +118,7 @@ __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
                 hit = ttProbe(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], WHITE), (int)d->depth, &v);
                 a.known[i] = hit ? v + 1 : 0;
             }
-            c = hit ? 0 : countMoves<WHITE>(a.boards[i], a.states[i]);
+            c = hit ? 0 : countChildren<WHITE>(a.boards[i], a.states[i]);
             a.cnt[i] = c;
         }
         if (d->useTT) bttCount(d, i < a.n, hit, false);
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
                 hit = ttProbe(d->tt, d->ttMask, hashPosition(q, st, 1 - C), (int)d->depth - L, &v);
                 a.known[i] = hit ? v + 1 : 0;
             }
-            c = hit ? 0 : countMoves<1 - C>(q, st);
+            c = hit ? 0 : countChildren<1 - C>(q, st);
             a.cnt[i] = c;
         }
         u64 s = blockSum64(c);
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
             makeMove<C>(q, st, aMv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            cnt = countMoves<1 - C>(q, st);
+            cnt = countChildren<1 - C>(q, st);
             root = aRoot[e];
         }
         const u32 r0 = __shfl(root, 0);  // lane 0 is always valid
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
             sb[w][lane] = q;
             ss[w][lane] = st;
             hit = ttProbe(d->tt, d->ttMask, hashPosition(q, st, 1 - C), 2, &hv);
-            cnt = hit ? 0 : countMoves<1 - C>(q, st);
+            cnt = hit ? 0 : countChildren<1 - C>(q, st);
         }
         if (lane < B_CHUNK) sacc[w][lane] = 0;
         u32 incl = waveInclScan(cnt);

@@ -452,11 +452,22 @@ PG_HD BB epSources(const Frame &f, u8 state)
 }
 
 // ------------------------------------------------------------- move counting
-// Number of legal moves (perft(1)).  Set-wise: the eight slider directions
-// are filled once for the enemy's attacks and our moves together; pawns are
-// counted by shifted sets, knights by one table row each.
-template <int C>
-PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
+// Number of legal moves.  Set-wise: the eight slider directions are filled
+// once for the enemy's attacks and our moves together; pawns are counted by
+// shifted sets, knights by one table row each.
+//
+// EXACT = true gives |generateMoves| (used to size a BFS level or a pair
+// list).  EXACT = false is the reference's countMoves, the perft(1) every
+// count of the reference's -cpu path ends in: out of check it counts at most
+// ONE en-passant evasion however many pawns can capture, because it ORs both
+// capture directions onto the single target square before the popcount
+// (countMovesOutOfCheck, MoveGeneratorBitboard.h:1639-1649), while its
+// generator emits one evasion per capturing pawn (:1085-1090).  The two differ
+// only when the side to move is in check from a pawn that just double-pushed
+// between two of its pawns (221,239 times among the perft(10) leaves of
+// Position 3, whose published count includes it).
+template <int C, bool EXACT>
+PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state)
 {
     Frame f = framePins<C>(q);
     const Sets &s = f.s;
@@ -517,13 +528,33 @@ PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
     n += (noCheck && (cr & 1) && !((s.occ | att) & ksEmpty)) ? 1 : 0;
     n += (noCheck && (cr & 2) && !(s.occ & qsEmpty) && !(att & qsSafe)) ? 1 : 0;
 
-    if (state >> 4) n += popc(epSources<C>(f, state));
+    if (state >> 4) {
+        BB eps = epSources<C>(f, state);
+        n += (!EXACT && f.nCheck) ? (eps ? 1u : 0u) : (u32)popc(eps);
+    }
     return n;
+}
+
+// the reference's perft(1) count (leaves of every perft)
+template <int C>
+PG_HD u32 countMoves(const QuadBitBoard &q, u8 state)
+{
+    return countMovesT<C, false>(q, state);
+}
+// the number of moves generateMoves emits (sizes of BFS levels and pair lists)
+template <int C>
+PG_HD u32 countChildren(const QuadBitBoard &q, u8 state)
+{
+    return countMovesT<C, true>(q, state);
 }
 
 PG_HD u32 countMoves(const QuadBitBoard &q, u8 state, int color)
 {
     return color == WHITE ? countMoves<WHITE>(q, state) : countMoves<BLACK>(q, state);
+}
+PG_HD u32 countChildren(const QuadBitBoard &q, u8 state, int color)
+{
+    return color == WHITE ? countChildren<WHITE>(q, state) : countChildren<BLACK>(q, state);
 }
 
 // --------------------------------------------------------- move generation

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(BS) void k_makecount(const QuadBitBoard *__restrict
         makeMove<C>(q, st, mv[i]);
         ob[i] = q;
         os[i] = st;
-        c = countMoves<1 - C>(q, st);
+        c = countChildren<1 - C>(q, st);
         cnt[i] = c;
     }
     u64 s = blockSum64(c);
@@ -251,7 +251,7 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
             makeMove<C>(q, st, mv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            cnt = countMoves<1 - C>(q, st);
+            cnt = countChildren<1 - C>(q, st);
         }
         u32 incl = waveInclScan(cnt);
         u32 off = incl - cnt;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(BS) void k_dedup_count(const QuadBitBoard *__restri
             bool dup = e.key == h.hi && e.idx && j != i && j < n && s[j] == st && b[j].bb[0] == q.bb[0] &&
                        b[j].bb[1] == q.bb[1] && b[j].bb[2] == q.bb[2] && b[j].bb[3] == q.bb[3];
             if (dup) r = (u32)j;
-            else c = countMoves<C>(q, st);
+            else c = countChildren<C>(q, st);
         }
         redir[i] = r;
         cnt[i] = c;
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard 
             sb[w][lane] = q;
             ss[w][lane] = st;
             hit = tt && ttProbe(tt, mask, hashPosition(q, st, 1 - C), 2, &hv);
-            cnt = hit ? 0 : countMoves<1 - C>(q, st);
+            cnt = hit ? 0 : countChildren<1 - C>(q, st);
         }
         if (lane < L2_CHUNK) sacc[w][lane] = 0;
         u32 incl = waveInclScan(cnt);

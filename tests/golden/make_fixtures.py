@@ -19,6 +19,9 @@ Configs (sizes and seeds as the benchmark defines them):
   c4  seeds 100..163, 10 plies, depth 8 (golden at depth 5)
   c5  request ids 0..9999: plies U[0,60] then depth U[3,7]; golden for every
       request of depth <= 5 and the first 64 of depth 6
+  ep  hand-set positions where a side in check by a double-pushed pawn has two
+      en-passant captures (which the reference's perft(1) counts once), and
+      positions one ply before them, perft 1..6
 """
 from __future__ import annotations
 
@@ -35,6 +38,15 @@ sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 
 
+EP_EVASION = [
+    ("wk-e4 checked by d5, c5/e5 take", "4k3/8/8/2PpP3/4K3/8/8/8 w - d6 0 1"),
+    ("bk-d5 checked by e4, d4/f4 take", "8/8/8/3k4/3pPp2/8/8/4K3 b - e3 0 1"),
+    ("before d7-d5+", "4k3/3p4/8/2P1P3/4K3/8/8/8 b - - 0 1"),
+    ("before e2-e4+", "8/8/8/3k4/3p1p2/8/4P3/4K3 w - - 0 1"),
+    ("full board, before d7-d5+", "rnbqkbnr/ppp1pppp/8/2P1P3/4K3/8/PP1P1PPP/RNBQ1BNR b kq - 0 1"),
+]
+
+
 def dump(name, rows):
     path = os.path.join(HERE, name)
     with open(path, "w") as f:
@@ -49,7 +61,7 @@ def main():
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     T = a.threads
-    only = set(a.only.split(",")) if a.only else {"c1", "c2", "c3", "c4", "c5"}
+    only = set(a.only.split(",")) if a.only else {"c1", "c2", "c3", "c4", "c5", "ep"}
     t0 = time.time()
 
     if "c1" in only:
@@ -86,6 +98,16 @@ def main():
         gold = dict(zip(pick, counts))
         dump("c5.jsonl", [{"seed": s, "plies": r[1], "fen": oracle.write_fen(r[0]), "depth": r[2],
                            **({"count": gold[s]} if s in gold else {})} for s, r in enumerate(reqs)])
+    if "ep" in only:
+        rows = []
+        for name, fen in EP_EVASION:
+            p = oracle.parse_fen(fen)
+            rows.append({"name": name, "fen": fen, "counts": [oracle.perft(p, d, T) for d in range(1, 7)]})
+        path = os.path.join(HERE, "ep_evasion.json")
+        with open(path, "w") as f:
+            json.dump({"_source": "oracle/oracle.c perft, whose perft(1) counts a checked side's en-passant "
+                                  "captures once (MoveGeneratorBitboard.h:1639-1649)", "positions": rows}, f, indent=1)
+        print(f"wrote {path}", flush=True)
     print(f"done in {time.time() - t0:.0f}s", flush=True)
 
 

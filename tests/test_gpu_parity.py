@@ -49,6 +49,40 @@ def test_leaf2_variants(gpu, variant):
         gpu.lib().pg_set_leaf2_variant(0)
 
 
+@pytest.mark.parametrize("leaf", [1, 2])
+def test_ep_evasion_positions(gpu, leaf):
+    # two en-passant captures out of a pawn check: levels are sized with both
+    # moves, and the last ply counts them once, as the reference's perft(1) does
+    gpu.set_leaf_plies(leaf)
+    try:
+        for p in load_json("ep_evasion.json")["positions"]:
+            pos = gpu.parse_fen(p["fen"])
+            print(f"[gpu] ep-evasion leaf={leaf} {p['name']}", flush=True)
+            assert [gpu.perft_bfs(pos, d) for d in range(1, 7)] == p["counts"], p["name"]
+            assert gpu.perft(pos, 6, launch_depth=3) == p["counts"][5], p["name"]
+    finally:
+        gpu.set_leaf_plies(2)
+
+
+def test_ep_evasion_level_kernels(gpu):
+    # the leaf entry point counts a checked side's en-passant captures once
+    p = gpu.parse_fen("4k3/3p4/8/2P1P3/4K3/8/8/8 b - - 0 1")
+    mv = [m for m in gpu.gen_moves(p) if (m & 63, (m >> 6) & 63) == (51, 35)]  # d7-d5+
+    assert len(mv) == 1
+    child = gpu.make_move(p, mv[0])
+    assert len(gpu.gen_moves(child)) == 9 and gpu.count_moves(child) == 8
+    pb = _d(gpu, np.array([list(p.bb)], dtype=np.uint64))
+    ps = _d(gpu, np.array([p.state], dtype=np.uint8))
+    di = _d(gpu, np.zeros(1, dtype=np.uint32))
+    dm = _d(gpu, np.array(mv, dtype=np.uint16))
+    want = {1: 8, 2: sum(gpu.count_moves(gpu.make_move(child, m)) for m in gpu.gen_moves(child))}
+    for plies in (1, 2):
+        s = gpu.DeviceArray(8).zero()
+        assert gpu.lib().pg_leaf(pb.ptr, ps.ptr, di.ptr, dm.ptr, 1, 1, plies, s.ptr, None) == 0
+        gpu.device_synchronize()
+        assert int(s.to_numpy(np.uint64, 1)[0]) == want[plies]
+
+
 def test_config1_all_depths(gpu):
     r = load_jsonl("c1.jsonl")[0]
     p = gpu.parse_fen(r["fen"])
@@ -162,9 +196,11 @@ def test_makecount_and_genmoves_match_host(gpu):
     gpu.device_synchronize()
     got_b = ob.to_numpy(np.uint64, 4 * n).reshape(n, 4)
     got_c = cnt.to_numpy(np.uint32, n)
+    # the level count sizes the next level: every generated move, which is the
+    # perft(1) count except for a checked side's double en-passant capture
     for k, ch in enumerate(children):
         assert tuple(int(x) for x in got_b[k]) == tuple(ch.bb)
-        assert got_c[k] == gpu.count_moves(ch)
+        assert got_c[k] == len(gpu.gen_moves(ch))
     assert int(tiles.to_numpy(np.uint32, (n + 255) // 256).astype(np.int64).sum()) == int(got_c.sum())
     # genmoves over the children (black to move)
     incl = _d(gpu, np.cumsum(got_c).astype(np.uint32))
@@ -212,7 +248,7 @@ def test_genmoves_writes_nothing_for_zero_count_entries(gpu):
     ps = [gpu.random_position(s, 10 + s % 20) for s in range(600)]
     ps = [p for p in ps if p.color == 0]
     n = len(ps)
-    counts = np.array([gpu.count_moves(p) for p in ps], dtype=np.uint32)
+    counts = np.array([len(gpu.gen_moves(p)) for p in ps], dtype=np.uint32)
     counts[::3] = 0
     incl = np.cumsum(counts).astype(np.uint32)
     c = int(incl[-1])

@@ -64,6 +64,21 @@ def test_batch_matches_golden_c3(gpu, leaf):
     assert got == [r["count"] for r in rows]
 
 
+@pytest.mark.parametrize("leaf", [1, 2])
+def test_batch_ep_evasions(gpu, leaf):
+    # every depth of the en-passant-evasion positions in one batch, TT off and on
+    rows = load_json("ep_evasion.json")["positions"]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows for _ in range(6)]
+    ds = [d for _ in rows for d in range(1, 7)]
+    want = [c for r in rows for c in r["counts"]]
+    assert gpu.perft_batch(ps, ds, leaf_plies=leaf) == want
+    gpu.tt_init(256, 0)
+    try:
+        assert gpu.perft_batch(ps, ds, leaf_plies=leaf, use_tt=True) == want
+    finally:
+        gpu.tt_free()
+
+
 def test_batch_graph_replay_and_eager_agree(gpu):
     rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (3, 4)][:600]
     ps = [gpu.parse_fen(r["fen"]) for r in rows]

@@ -88,6 +88,14 @@ def test_cpu_perft_matches_known(engine, depth):
             assert engine.perft_cpu(engine.parse_fen(p["fen"]), depth) == p["counts"][depth - 1]
 
 
+def test_cpu_perft_ep_evasions(engine, orc):
+    # a side in check with two en-passant captures: perft(1) counts them once
+    for p in load_json("ep_evasion.json")["positions"]:
+        e = engine.parse_fen(p["fen"])
+        assert [engine.perft_cpu(e, d) for d in range(1, 6)] == p["counts"][:5], p["name"]
+        assert engine.count_moves(e) == orc.count_moves(orc.parse_fen(p["fen"]))
+
+
 def test_cpu_perft_with_host_tt_matches(engine):
     fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
     engine.tt_init(0, 64)

@@ -2,9 +2,11 @@
 
 Checks: published perft counts (known_positions.json), the fast legal
 generator against the independent brute-force generator (pseudo-legal moves
-filtered by king safety) on seeded random positions, count == |generate| on
-every node of full trees, FEN round trips, and that a planted mistake in the
-oracle's inputs is caught by these checks.
+filtered by king safety) on seeded random positions, the count against the
+brute generator's count under the reference's perft(1) convention (a side in
+check counts its en-passant captures once) on every node of full trees, FEN
+round trips, and that a planted mistake in the oracle's inputs is caught by
+these checks.
 """
 import pytest
 
@@ -40,7 +42,7 @@ def test_fast_equals_brute_on_random_positions(orc):
         p = orc.random_position(seed, seed % 90, require_nonterminal=False)
         fast, brute = orc.gen_moves(p), orc.brute_gen_moves(p)
         assert _canon(fast) == _canon(brute), (seed, orc.write_fen(p))
-        assert orc.count_moves(p) == len(fast)
+        assert orc.count_moves(p) == orc.brute_count_moves(p)
 
 
 def test_fast_equals_brute_perft3(orc):
@@ -57,9 +59,32 @@ def test_count_equals_generate_on_tree(orc, fen):
         nxt = []
         for p in frontier:
             mv = orc.gen_moves(p)
-            assert orc.count_moves(p) == len(mv)
+            assert orc.count_moves(p) == orc.brute_count_moves(p) <= len(mv)
             nxt.extend(orc.make_move(p, m) for m in mv)
         frontier = nxt[:4000]
+
+
+# A side in check by a pawn that just made a double push, with two pawns that
+# can take it en passant: both captures are legal moves, and the reference's
+# countMovesOutOfCheck counts them as one (MoveGeneratorBitboard.h:1639-1649).
+EP_DOUBLE_EVASION = [
+    # white king e4 checked by d7-d5; c5xd6 and e5xd6 both legal
+    ("4k3/8/8/2PpP3/4K3/8/8/8 w - d6 0 1", 9, 2),
+    # black king d5 checked by e2-e4; d4xe3 and f4xe3 both legal
+    ("8/8/8/3k4/3pPp2/8/8/4K3 b - e3 0 1", 9, 2),
+]
+
+
+@pytest.mark.parametrize("fen,legal,ep", EP_DOUBLE_EVASION)
+def test_count_takes_ep_evasions_once(orc, fen, legal, ep):
+    p = orc.parse_fen(fen)
+    mv = orc.gen_moves(p)
+    assert len(mv) == legal == len(orc.brute_gen_moves(p))
+    assert sum((m >> 12) == 5 for m in mv) == ep
+    assert orc.count_moves(p) == orc.brute_count_moves(p) == legal - ep + 1
+    assert orc.perft(p, 1) == legal - ep + 1
+    # one ply deeper every leaf is counted, both captures included
+    assert orc.perft(p, 2) == sum(orc.count_moves(orc.make_move(p, m)) for m in mv)
 
 
 def test_fen_round_trip(orc):
