@@ -230,54 +230,23 @@ __global__ __launch_bounds__(BS) void k_b_expand(BatchDev *d, int L)
     if (d->oom) return;
     const LvlDev a = d->lv[L];
     const LvlDev b = d->lv[L + 1];
-    constexpr int PER = EXPAND_TILE / BS;
-    __shared__ u32 tile[EXPAND_TILE];
+    __shared__ __attribute__((aligned(16))) u32 tile[EXPAND_TILE];
     __shared__ u32 wmax[BS / 64];
     const u64 c = b.n;
     const u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
-    const u32 *incl = a.cnt;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         const u64 c0 = t * EXPAND_TILE;
-        const u64 p0 = a.parts[t], p1 = a.parts[t + 1];
-        for (int k = tid; k < EXPAND_TILE; k += BS) tile[k] = 0;
+        for (int k = threadIdx.x; k < EXPAND_TILE; k += BS) tile[k] = 0;
         __syncthreads();
-        for (u64 p = p0 + tid; p <= p1; p += BS) {
-            u64 ex = p ? incl[p - 1] : 0, in = incl[p];
-            if (in > ex) {
-                u64 st = ex > c0 ? ex - c0 : 0;
-                if (st < EXPAND_TILE) tile[st] = (u32)p + 1;
-            }
-        }
+        u32 o[EXPAND_PER], r[EXPAND_PER];
+        expandTileParents(a.cnt, a.parts[t], a.parts[t + 1], c0, tile, wmax, o);
+        // each child's root: the parent's root (level 1 entries are roots' children)
+        const u64 base = c0 + (u64)threadIdx.x * EXPAND_PER;
+#pragma unroll
+        for (int k = 0; k < EXPAND_PER; k++) r[k] = (L == 0 || base + k >= c) ? o[k] : a.root[o[k]];
+        storeTileStreaming(tile, o, b.idx, c0, c);
         __syncthreads();
-        u32 v[PER];
-        u32 m = 0;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            u32 x = tile[tid * PER + k];
-            m = x > m ? x : m;
-            v[k] = m;
-        }
-        u32 wm = m;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            u32 tt = __shfl_up(wm, o);
-            if (lane >= o) wm = tt > wm ? tt : wm;
-        }
-        if (lane == 63) wmax[w] = wm;
-        u32 prev = __shfl_up(wm, 1);
-        if (lane == 0) prev = 0;
-        __syncthreads();
-        for (int k = 0; k < w; k++) prev = wmax[k] > prev ? wmax[k] : prev;
-        const u64 base = c0 + (u64)tid * PER;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            if (base + k < c) {
-                u32 parent = (v[k] > prev ? v[k] : prev) - 1;
-                b.idx[base + k] = parent;
-                b.root[base + k] = L == 0 ? parent : a.root[parent];
-            }
-        }
+        storeTileStreaming(tile, r, b.root, c0, c);
         __syncthreads();  // the tile is cleared again by the next iteration
     }
 }

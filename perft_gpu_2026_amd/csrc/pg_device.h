@@ -115,6 +115,78 @@ __device__ __forceinline__ void scanApplyTile(u32 *__restrict__ d, u64 n, const 
     }
 }
 
+// ------------------------------------------------------- expand (one tile)
+// Parents of the children c0 .. c0+EXPAND_TILE-1, given the parents p0..p1
+// whose child ranges meet the tile (incl = inclusive child counts).  Each
+// parent with children writes its index + 1 at the LDS position of its first
+// child (the parent holding the tile's first child at position 0); an
+// inclusive max-scan then gives every child its parent.  Thread t returns the
+// parents of children c0 + 8t .. c0 + 8t + 7 in o[].  `tile` must hold zeros
+// on entry; every thread's reads of it are done when this returns.
+constexpr int EXPAND_PER = EXPAND_TILE / BS;
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void expandTileParents(const u32 *__restrict__ incl, u64 p0, u64 p1, u64 c0,
+                                                  u32 *__restrict__ tile, u32 *__restrict__ wmax,
+                                                  u32 (&o)[EXPAND_PER])
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (u64 p = p0 + tid; p <= p1; p += BS) {
+        u64 ex = p ? incl[p - 1] : 0, in = incl[p];
+        if (in > ex) {
+            u64 st = ex > c0 ? ex - c0 : 0;
+            if (st < EXPAND_TILE) tile[st] = (u32)p + 1;
+        }
+    }
+    __syncthreads();
+    u32 v[EXPAND_PER];
+    u32 m = 0;
+#pragma unroll
+    for (int k = 0; k < EXPAND_PER; k++) {
+        u32 x = tile[tid * EXPAND_PER + k];
+        m = x > m ? x : m;
+        v[k] = m;
+    }
+    // running max across the wave, then across the block's waves
+    u32 wm = m;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        u32 t = __shfl_up(wm, d);
+        if (lane >= d) wm = t > wm ? t : wm;
+    }
+    if (lane == 63) wmax[w] = wm;
+    u32 prev = __shfl_up(wm, 1);
+    if (lane == 0) prev = 0;
+    __syncthreads();
+    for (int k = 0; k < w; k++) prev = wmax[k] > prev ? wmax[k] : prev;
+#pragma unroll
+    for (int k = 0; k < EXPAND_PER; k++) o[k] = (v[k] > prev ? v[k] : prev) - 1;
+}
+
+// Writes one tile's values (thread t holding entries 8t .. 8t+7 in o[]) to
+// dst[c0 .. min(c0 + EXPAND_TILE, c)).  The values go through LDS so that each
+// store instruction of a wave covers 1 KB of consecutive entries, written with
+// non-temporal 16-byte stores: the output is far larger than the caches and is
+// read back by the next level's kernels, not by this one.  No thread may
+// still read `tile` on entry; the caller places a barrier before reusing it.
+__device__ __forceinline__ void storeTileStreaming(u32 *__restrict__ tile, const u32 (&o)[EXPAND_PER],
+                                                   u32 *__restrict__ dst, u64 c0, u64 c)
+{
+    const int tid = threadIdx.x;
+    u32x4 *t4 = (u32x4 *)tile;
+#pragma unroll
+    for (int k = 0; k < EXPAND_PER / 4; k++)
+        t4[tid * (EXPAND_PER / 4) + k] = u32x4{o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]};
+    __syncthreads();
+    if (c0 + EXPAND_TILE <= c && ((uintptr_t)(dst + c0) & 15) == 0) {
+        u32x4 *d4 = (u32x4 *)(dst + c0);
+#pragma unroll
+        for (int k = 0; k < EXPAND_PER / 4; k++) __builtin_nontemporal_store(t4[k * BS + tid], d4 + k * BS + tid);
+    } else {
+        for (int k = tid; k < EXPAND_TILE; k += BS)
+            if (c0 + k < c) dst[c0 + k] = tile[k];
+    }
+}
+
 // first p in [0, n) with incl[p] > c (n if none)
 __device__ __forceinline__ u64 upperBound(const u32 *__restrict__ incl, u64 n, u64 c)
 {

@@ -8,7 +8,7 @@
 //   k_scan_apply  wave64 block scan + tile offset -> inclusive counts (HBM)
 //   k_partitions  one upper_bound per 2048-child tile
 //   k_expand      parent index of every child: first-child scatter + max-scan
-//                 in LDS, coalesced 16-byte stores                 (HBM)
+//                 in LDS, coalesced non-temporal 16-byte stores    (HBM)
 //   k_genmoves    moves of every board, staged in LDS, written coalesced
 // and after the last level a leaf kernel sums perft(1) or perft(2) of every
 // entry (k_leaf; the 2-ply form is the reference's fused leaf, here with the
@@ -84,64 +84,21 @@ __global__ __launch_bounds__(BS) void k_partitions(const u32 *__restrict__ incl,
     parts[t] = (u32)(p < n ? p : n - 1);
 }
 
-// Parent index of every child of one 2048-child tile.  Each parent whose
-// first child falls in the tile writes its index (+1) at that position of an
-// LDS tile (zero elsewhere; the parent holding the tile's first child writes
-// position 0); an inclusive max-scan then fills every child with its parent.
-// Per child: one LDS write (the clear), one read, a max, one coalesced 4-byte
-// store -- no per-child search.
+// Parent index of every child of one 2048-child tile (first-child scatter and
+// max-scan in LDS, then streaming 16-byte stores).  Per child: one LDS clear,
+// one read, a max, a 4-byte LDS round trip and 4 bytes of a coalesced store --
+// no per-child search.
 __global__ __launch_bounds__(BS) void k_expand(const u32 *__restrict__ incl, u64 n, u64 c,
                                                const u32 *__restrict__ parts, u32 *__restrict__ parentIdx)
 {
-    constexpr int PER = EXPAND_TILE / BS;  // 8 consecutive children per thread
-    __shared__ u32 tile[EXPAND_TILE];
+    __shared__ __attribute__((aligned(16))) u32 tile[EXPAND_TILE];
     __shared__ u32 wmax[BS / 64];
     const u64 c0 = (u64)blockIdx.x * EXPAND_TILE;
-    const u64 p0 = parts[blockIdx.x], p1 = parts[blockIdx.x + 1];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int k = tid; k < EXPAND_TILE; k += BS) tile[k] = 0;
+    for (int k = threadIdx.x; k < EXPAND_TILE; k += BS) tile[k] = 0;
     __syncthreads();
-    for (u64 p = p0 + tid; p <= p1; p += BS) {
-        u64 ex = p ? incl[p - 1] : 0, in = incl[p];
-        if (in > ex) {
-            u64 st = ex > c0 ? ex - c0 : 0;
-            if (st < EXPAND_TILE) tile[st] = (u32)p + 1;
-        }
-    }
-    __syncthreads();
-    u32 v[PER];
-    u32 m = 0;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        u32 x = tile[tid * PER + k];
-        m = x > m ? x : m;
-        v[k] = m;
-    }
-    // running max across the wave, then across the block's waves
-    u32 wm = m;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        u32 t = __shfl_up(wm, o);
-        if (lane >= o) wm = t > wm ? t : wm;
-    }
-    if (lane == 63) wmax[w] = wm;
-    u32 prev = __shfl_up(wm, 1);
-    if (lane == 0) prev = 0;
-    __syncthreads();
-    for (int k = 0; k < w; k++) prev = wmax[k] > prev ? wmax[k] : prev;
-    const u64 base = c0 + (u64)tid * PER;
-    u32 o[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) o[k] = (v[k] > prev ? v[k] : prev) - 1;
-    if (base + PER <= c && ((uintptr_t)parentIdx & 15) == 0) {
-        uint4 *dst = (uint4 *)(parentIdx + base);
-#pragma unroll
-        for (int k = 0; k < PER / 4; k++) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < PER; k++)
-            if (base + k < c) parentIdx[base + k] = o[k];
-    }
+    u32 o[EXPAND_PER];
+    expandTileParents(incl, parts[blockIdx.x], parts[blockIdx.x + 1], c0, tile, wmax, o);
+    storeTileStreaming(tile, o, parentIdx, c0, c);
 }
 
 template <int C>
