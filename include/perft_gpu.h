@@ -160,9 +160,11 @@ int pg_level_makecount(const void *parentBoards, const uint8_t *parentStates, co
                        const uint16_t *moves, int color, uint64_t n, void *boards, uint8_t *states,
                        uint32_t *counts, uint32_t *tileSums, void *stream);
 /* cub::DeviceScan::InclusiveSum replacement, perft_kernels.cu:786: in-place
- * inclusive scan of n u32 with a wave64 block scan.  If tileSums is NULL it
- * is computed here.  scratch must hold 8*(ceil(n/256)+1) bytes; *total (host
- * pointer, may be NULL) receives the 64-bit sum (synchronises the stream). */
+ * inclusive scan of n u32 with a wave64 block scan.  tileSums[t] is the sum
+ * of data[256t .. 256t+255]; if it is NULL the sums are computed here.  scratch
+ * must hold 8*(ceil(n/256)+1) bytes, plus 4*ceil(n/256) more when tileSums is
+ * NULL; *total (host pointer, may be NULL) receives the 64-bit sum
+ * (synchronises the stream). */
 int pg_scan_u32(uint32_t *data, uint64_t n, const uint32_t *tileSums, void *scratch, uint64_t *total, void *stream);
 /* merge_path_partitions + mp_interval_expand_kernel, perft_kernels.cu:77-173:
  * parentIdx[j] = #{p < n : incl[p] <= j} for j < c.  scratch: 4*(ceil(c/2048)+1) B */

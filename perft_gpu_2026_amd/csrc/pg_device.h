@@ -56,31 +56,75 @@ __device__ __forceinline__ u64 blockSum64(u64 v)
 }
 
 // Exclusive scan of `tiles` tile sums into tileOffs[0..tiles], tileOffs[tiles]
-// = total, by one block of TOP_BS threads: each thread sums a run of
-// consecutive tiles, the block scans the run totals, and each thread writes
-// its run's offsets (one barrier, whatever the number of tiles).
+// = total, by one block of TOP_BS threads.  The sums are taken in rows of
+// 4 * TOP_BS: thread t reads sums 4t..4t+3 of a row with one 16-byte load, so
+// each wave load covers 1 KB of consecutive sums; a wave scan of the thread
+// totals and the waves' totals in LDS give every sum its offset.  TOP_ROWS
+// rows share one barrier and have their loads in flight together.
 constexpr int TOP_BS = 1024;
+constexpr int TOP_ROWS = 8;
 __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u64 tiles, u64 *__restrict__ tileOffs)
 {
-    __shared__ u64 wtot[TOP_BS / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const u64 run = (tiles + TOP_BS - 1) / TOP_BS;
-    const u64 b0 = (u64)threadIdx.x * run;
-    const u64 b1 = b0 + run < tiles ? b0 + run : tiles;
-    u64 local = 0;
-#pragma unroll 8
-    for (u64 i = b0; i < b1; i++) local += tileSums[i];
-    u64 incl = waveInclScan64(local);
-    if (lane == 63) wtot[w] = incl;
-    __syncthreads();
-    u64 acc = incl - local;
-    for (int k = 0; k < w; k++) acc += wtot[k];
-#pragma unroll 8
-    for (u64 i = b0; i < b1; i++) {
-        tileOffs[i] = acc;
-        acc += tileSums[i];
+    __shared__ u32 wtot[TOP_ROWS][TOP_BS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    constexpr u64 ROW = 4ull * TOP_BS, PASS = ROW * TOP_ROWS;
+    const bool aligned = ((uintptr_t)tileSums & 15) == 0;
+    u64 carry = 0;
+    for (u64 c0 = 0; c0 < tiles; c0 += PASS) {
+        const u64 left = tiles - c0;
+        const int rows = left >= PASS ? TOP_ROWS : (int)((left + ROW - 1) / ROW);  // uniform
+        u32 x[TOP_ROWS][4], ex[TOP_ROWS];
+#pragma unroll
+        for (int r = 0; r < TOP_ROWS; r++) {
+            const u64 i0 = c0 + r * ROW + 4u * tid;
+            x[r][0] = x[r][1] = x[r][2] = x[r][3] = 0;
+            if (r < rows) {
+                if (aligned && i0 + 4 <= tiles) {
+                    uint4 q = *(const uint4 *)(tileSums + i0);
+                    x[r][0] = q.x, x[r][1] = q.y, x[r][2] = q.z, x[r][3] = q.w;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (i0 + k < tiles) x[r][k] = tileSums[i0 + k];
+                }
+            }
+        }
+        // a row holds at most 4096 tile sums of at most 256 * 218 each: u32
+#pragma unroll
+        for (int r = 0; r < TOP_ROWS; r++) {
+            if (r < rows) {
+                u32 t = x[r][0] + x[r][1] + x[r][2] + x[r][3];
+                u32 incl = waveInclScan(t);
+                ex[r] = incl - t;
+                if (lane == 63) wtot[r][w] = incl;
+            }
+        }
+        __syncthreads();
+        u64 rowBase = carry;
+#pragma unroll
+        for (int r = 0; r < TOP_ROWS; r++) {
+            if (r < rows) {
+                u32 before = 0, all = 0;
+#pragma unroll
+                for (int k = 0; k < TOP_BS / 64; k++) {
+                    u32 t = wtot[r][k];
+                    all += t;
+                    before += k < w ? t : 0u;
+                }
+                const u64 i0 = c0 + r * ROW + 4u * tid;
+                u64 o = rowBase + before + ex[r];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (i0 + k < tiles) tileOffs[i0 + k] = o;
+                    o += x[r][k];
+                }
+                rowBase += all;
+            }
+        }
+        carry = rowBase;
+        __syncthreads();  // wtot is rewritten by the next pass
     }
-    if (threadIdx.x == TOP_BS - 1) tileOffs[tiles] = acc;
+    if (tid == 0) tileOffs[tiles] = carry;
 }
 
 // One wave turns the 256 counts of `tile` into inclusive counts: lane l holds
