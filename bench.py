@@ -142,15 +142,19 @@ def tt_leg(pg) -> dict:
             "checked": "depth-5 oracle counts; TT on == TT off at depth 7 (all 64) and depth 8 (first 4)"}
 
 
-# The reference's own published single-position runs (no TT), for context:
-# README.md:51-52 (RTX PRO 6000 Blackwell) and :74 (RTX 4090).
+# The reference's own published single-position runs, for context:
+# README.md:51-52 and :58-60 (RTX PRO 6000 Blackwell) and :74 (RTX 4090).
 REF_RUNS = [("startpos", 9, "1.867 s on RTX PRO 6000"), ("kiwipete", 7, "0.177 s on RTX PRO 6000"),
             ("position3", 10, "1.55 s on RTX 4090")]
+REF_RUNS_TT = [("startpos", 9, "0.174 s on RTX PRO 6000"), ("startpos", 10, "2.040 s on RTX PRO 6000"),
+               ("startpos", 11, "24.33 s on RTX PRO 6000")]
 
 
-def ref_positions_leg(pg) -> dict:
-    """The reference README's published positions through the launcher (no TT),
-    each count checked against the published value."""
+def ref_positions_leg(pg, tt_max_depth: int) -> dict:
+    """The reference README's published positions through the launcher, each
+    count checked against the published value: without the tables, then
+    startpos with them (64 GB device table, 32 GB host tier, cleared before
+    each run)."""
     known = {p["name"]: p for p in json.load(open(os.path.join(ROOT, "tests", "golden", "known_positions.json")))["positions"]}
     out = {}
     for name, d, theirs in REF_RUNS:
@@ -163,6 +167,24 @@ def ref_positions_leg(pg) -> dict:
         dt = time.perf_counter() - t0
         assert c == want, (name, d, c, want)
         out[f"{name}_perft{d}"] = {"nodes": c, "seconds": dt, "nodes_per_s": c / dt, "reference_published": theirs}
+    pg.tt_init(65536, 32768)
+    try:
+        for name, d, theirs in REF_RUNS_TT:
+            if d > tt_max_depth:
+                continue
+            k = known[name]
+            want = k["counts"][d - 1]
+            p = pg.parse_fen(k["fen"])
+            pg.lib().pg_tt_clear()
+            print(f"[bench] {name} perft {d} with the tables", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            c = pg.perft(p, d, use_tt=True)
+            dt = time.perf_counter() - t0
+            assert c == want, (name, d, c, want)
+            out[f"{name}_perft{d}_tt"] = {"nodes": c, "seconds": dt, "nodes_per_s": c / dt,
+                                          "reference_published": theirs + ", TT"}
+    finally:
+        pg.tt_free()
     return out
 
 
@@ -261,6 +283,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched-vs-plain leg")
     ap.add_argument("--no-tt", action="store_true", help="skip the config-4 transposition-table leg")
     ap.add_argument("--no-ref-positions", action="store_true", help="skip the reference README positions")
+    ap.add_argument("--ref-tt-depth", type=int, default=11,
+                    help="deepest startpos perft run with the tables in the reference-positions leg (8 = none)")
     ap.add_argument("--serve", type=int, default=10000, help="config-5 requests to replay (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0)
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
@@ -339,7 +363,7 @@ def main():
         out["tt_config4"] = tt_leg(pg)
     if not a.no_ref_positions:
         print("[bench] reference README positions", file=sys.stderr, flush=True)
-        out["reference_positions"] = ref_positions_leg(pg)
+        out["reference_positions"] = ref_positions_leg(pg, a.ref_tt_depth)
     if a.serve:
         print("[bench] config-5 serving", file=sys.stderr, flush=True)
         out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate)

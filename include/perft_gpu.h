@@ -63,10 +63,12 @@ uint64_t pg_arena_bytes(void);
 /* caps the bytes of the arena a GPU call may use (0 = all of it); an injectable
  * arena size that lets a test force the OOM fallback (launcher.cu:600-610) */
 void pg_set_arena_limit(uint64_t bytes);
-/* launcher above the launch depth: 1 (default) expands the frontier on the host,
- * merges identical positions and runs them through the batched engine; 0 keeps
- * the reference's serial host recursion with one GPU call per node
- * (launcher.cu:572-715) */
+/* launcher above the launch depth without the TT: 1 (default) expands the
+ * frontier on the host, merges identical positions and runs them through the
+ * batched engine; 0 keeps the reference's serial host recursion with one GPU
+ * call per node (launcher.cu:572-715).  With the TT the recursion is always
+ * serial: each GPU call merges identical positions per level and the table
+ * carries subtrees between calls. */
 void pg_set_frontier_batch(int on);
 
 /* ========================= host helpers (util.cpp) ========================== */

@@ -298,7 +298,11 @@ int gpuBfs(const pg_position *pos, int depth, bool useTT, void *buf, uint64_t by
 // launch depth and makes one GPU call per node, launcher.cu:572-715): expand
 // `depth - ld` plies on the host, merge identical positions with their
 // multiplicities (exact: the whole position is the key), and run every unique
-// frontier position through the batched engine in one pass.
+// frontier position through the batched engine in one pass.  Without the
+// tables only: with them, each serial GPU call merges the identical positions
+// of every BFS level and stores whole subtrees for the next call, which on
+// transposition-rich trees is far less work (startpos perft 10: 1.9 s serial,
+// 48 s through the batched engine, whose levels are not merged).
 struct PosKey {
     u64 bb[4];
     u8 state, color;
@@ -317,7 +321,7 @@ struct PosKeyHash {
     }
 };
 
-static int frontierPerft(const pg_position *pos, int depth, int ld, bool useTT, u128 *out)
+static int frontierPerft(const pg_position *pos, int depth, int ld, u128 *out)
 {
     std::unordered_map<PosKey, u64, PosKeyHash> cur, next;
     PosKey k0;
@@ -361,7 +365,6 @@ static int frontierPerft(const pg_position *pos, int depth, int ld, bool useTT, 
     std::vector<u64> counts(ps.size());
     pg_batch_options opt{};
     opt.useGraph = 1;
-    opt.useTT = useTT;
     const size_t CH = 1 << 20;
     for (size_t b = 0; b < ps.size(); b += CH) {
         int n = (int)std::min(CH, ps.size() - b);
@@ -404,9 +407,9 @@ static int recurse(const pg_position *pos, int depth, bool useTT, u128 *out)
             total = u128(c);
         }
     }
-    if (split && g_frontierBatch && depth > 1 && depth - g_effectiveLD <= FRONTIER_MAX_PLIES) {
+    if (split && g_frontierBatch && !useTT && depth > 1 && depth - g_effectiveLD <= FRONTIER_MAX_PLIES) {
         int ld = std::min(g_effectiveLD, depth - 1);
-        PG_TRY(frontierPerft(pos, depth, ld < 1 ? 1 : ld, useTT, &total));
+        PG_TRY(frontierPerft(pos, depth, ld < 1 ? 1 : ld, &total));
     } else if (split) {
         if (depth <= 1) {
             // a GPU call of depth 1 cannot run out of memory; depth 0 is a leaf

@@ -39,7 +39,11 @@ def test_tt_bfs_equals_no_tt(tt, leaf):
 def test_tt_launcher_with_host_tier(tt):
     r = load_jsonl("c2.jsonl")[0]
     p = tt.parse_fen(r["fen"])
-    for ld in (3, 4):
+    # with the tables the launcher recurses serially above the launch depth;
+    # launch depth 4 leaves 19,494 host nodes after 3 plies, where moves of
+    # both sides transpose and the host tier hits
+    for ld in (4, 5):
+        print(f"[gpu] launcher tt ld={ld}", flush=True)
         assert tt.perft(p, 7, launch_depth=ld, use_tt=True) == r["counts"][6]
     st = tt.tt_stats()
     assert st[3] > 0 and st[4] > 0  # host tier probed and hit
