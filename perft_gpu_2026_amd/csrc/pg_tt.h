@@ -12,8 +12,12 @@
 // 16-B entries {lock, data}, data = count << 8 | depth, lock = key ^ data
 // (XOR lock: a torn 16-B read fails the check, so it reads as a miss; 128-bit
 // accesses are single dwordx4 instructions but not architecturally atomic).
-// Slot = mix(hash.lo, depth) & mask; key = hash.hi.  Depth-tagged so requests
-// of different depths share the table (serving).  Counts >= 2^56 are not stored.
+// Entries pair up in 32-B buckets, bucket = mix(hash.lo, depth); key =
+// hash.hi.  The bucket's first entry keeps the deeper of what it holds and
+// what is stored (its own key and ties replace it), the second takes every
+// other store, so the count of a deep subtree survives the flood of depth-2
+// leaf stores that an always-replace slot loses.  Depth-tagged so requests of
+// different depths share the table (serving).  Counts >= 2^56 are not stored.
 //
 // Host TT (tt.h:165-232): lossless chained table -- a bump pool of entries and
 // CAS-prepended bucket chains; never evicts, drops new stores once the pool is
@@ -50,9 +54,10 @@ PG_HD Hash128 hashPosition(const QuadBitBoard &q, u8 state, int color)
 // ---------------------------------------------------------------- device TT
 static constexpr u64 TT_MAX_COUNT = (1ULL << 56) - 1;
 
-PG_HD u64 ttSlot(const Hash128 &h, int depth, u64 mask)
+// first entry of the bucket (mask = entries - 1, entries a power of two >= 2)
+PG_HD u64 ttBucket(const Hash128 &h, int depth, u64 mask)
 {
-    return (h.lo ^ ((u64)depth * 0x9E3779B97F4A7C15ULL)) & mask;
+    return (h.lo ^ ((u64)depth * 0x9E3779B97F4A7C15ULL)) & mask & ~1ULL;
 }
 
 #if defined(__HIPCC__)
@@ -63,10 +68,12 @@ struct alignas(16) TTEntry {
 __device__ __forceinline__ bool ttProbe(const TTEntry *tt, u64 mask, const Hash128 &h, int depth, u64 *count)
 {
     if (!tt) return false;
-    TTEntry e = tt[ttSlot(h, depth, mask)];  // one 16-B load
-    bool hit = ((e.lock ^ e.data) == h.hi) && ((e.data & 0xFF) == (u64)depth);
-    *count = e.data >> 8;
-    return hit;
+    const TTEntry *b = tt + ttBucket(h, depth, mask);
+    TTEntry e0 = b[0], e1 = b[1];  // one 32-B bucket
+    bool h0 = ((e0.lock ^ e0.data) == h.hi) && ((e0.data & 0xFF) == (u64)depth);
+    bool h1 = ((e1.lock ^ e1.data) == h.hi) && ((e1.data & 0xFF) == (u64)depth);
+    *count = (h0 ? e0.data : e1.data) >> 8;
+    return h0 || h1;
 }
 
 __device__ __forceinline__ void ttStore(TTEntry *tt, u64 mask, const Hash128 &h, int depth, u64 count)
@@ -74,7 +81,12 @@ __device__ __forceinline__ void ttStore(TTEntry *tt, u64 mask, const Hash128 &h,
     if (!tt || count > TT_MAX_COUNT) return;
     u64 data = (count << 8) | (u64)depth;
     TTEntry e{h.hi ^ data, data};
-    tt[ttSlot(h, depth, mask)] = e;  // one 16-B store
+    TTEntry *b = tt + ttBucket(h, depth, mask);
+    TTEntry e0 = b[0];
+    u64 d0 = e0.data & 0xFF;
+    bool own0 = ((e0.lock ^ e0.data) == h.hi) && d0 == (u64)depth;
+    if (own0 || (u64)depth >= d0) b[0] = e;  // one 16-B store
+    else b[1] = e;
 }
 #endif
 
