@@ -188,7 +188,7 @@ def ref_positions_leg(pg, tt_max_depth: int) -> dict:
     return out
 
 
-def serve_leg(pg, n_req: int, rate: float) -> dict:
+def serve_leg(pg, n_req: int, rate: float, tt_gb: int) -> dict:
     import ctypes
     import random
     import threading
@@ -199,7 +199,7 @@ def serve_leg(pg, n_req: int, rate: float) -> dict:
     L = pg.lib()
 
     def run(arrivals):
-        cfg = pg.ServeConfig(200 << 30, 16 << 30, 8, 1)
+        cfg = pg.ServeConfig(200 << 30, tt_gb << 30, 8, 1)
         h = ctypes.c_void_p()
         if L.pg_serve_create(ctypes.byref(cfg), ctypes.byref(h)) != 0:
             raise RuntimeError(pg.last_error())
@@ -287,6 +287,8 @@ def main():
                     help="deepest startpos perft run with the tables in the reference-positions leg (8 = none)")
     ap.add_argument("--serve", type=int, default=10000, help="config-5 requests to replay (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0)
+    ap.add_argument("--serve-tt-gb", type=int, default=16,
+                    help="shared transposition table of the serving leg, inside its 200 GB budget")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -366,7 +368,7 @@ def main():
         out["reference_positions"] = ref_positions_leg(pg, a.ref_tt_depth)
     if a.serve:
         print("[bench] config-5 serving", file=sys.stderr, flush=True)
-        out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate)
+        out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate, a.serve_tt_gb)
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(fx["fen"], depth)
