@@ -242,7 +242,10 @@ int main(int argc, char *argv[])
     printf("Launch depth: %d\n", currentLD);
     fflush(stdout);
     if (useTT) {
-        uint64_t dMB = dttMB ? (uint64_t)dttMB : 16384, hMB = httMB ? (uint64_t)httMB : 4096;
+        // the reference gives its device tables most of the free memory
+        // (launcher.cu:146-160); 64 GB fits beside the default arena on a
+        // 288 GB device (the library halves it where it does not fit)
+        uint64_t dMB = dttMB ? (uint64_t)dttMB : 65536, hMB = httMB ? (uint64_t)httMB : 4096;
         if (pg_tt_init(dMB, hMB) != PG_OK) printf("warning: TT init failed: %s\n", pg_last_error());
     }
     int rc = 0;

@@ -113,12 +113,24 @@ static int deviceInit(uint64_t deviceMB)
     u64 slots = 1;
     while (slots * 2 * 16 <= bytes) slots *= 2;
     void *p = nullptr;
-    if (hipMalloc(&p, slots * 16) != hipSuccess) {
+    // a table that does not fit beside the arena is halved until it does
+    // (down to 1/16 of the request), as the reference's auto sizing takes
+    // what is free (launcher.cu:146-160)
+    const u64 want = slots;
+    while (hipMalloc(&p, slots * 16) != hipSuccess) {
         (void)hipGetLastError();
+        p = nullptr;
+        if (slots <= want / 16 || slots <= 65536) break;
+        slots >>= 1;
+    }
+    if (!p) {
         fprintf(stderr, "warning: device TT allocation of %llu MB failed, device TT disabled\n",
                 (unsigned long long)deviceMB);
         return PG_OK;
     }
+    if (slots != want)
+        fprintf(stderr, "warning: device TT of %llu MB does not fit, using %llu MB\n", (unsigned long long)deviceMB,
+                (unsigned long long)(slots * 16 >> 20));
     PG_HIP(hipMemsetAsync(p, 0, slots * 16, L.stream));
     PG_HIP(hipStreamSynchronize(L.stream));
     L.tt.entries = p;
