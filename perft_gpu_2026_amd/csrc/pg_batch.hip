@@ -19,8 +19,9 @@
 //    queue (an atomic cursor), so roots with huge subtrees and roots with tiny
 //    ones share the GPU without a tail; results go to per-root u64 counters
 //    with one atomic per wave and root;
-//  * an autotuner picks the leaf plies (1 or 2) per (depth, group size) by
-//    timing both on a sample of the group and caches the choice.
+//  * an autotuner picks the leaf plies (1 or 2) and, with the table on,
+//    whether a group uses it, per (depth, group size), by timing each choice
+//    on a sample of the group, and caches the choice.
 #include <string.h>
 
 #include <algorithm>
@@ -564,7 +565,7 @@ struct BatchState {
     BatchDev *dev = nullptr;  // device descriptor
     BatchDev host{};          // host copy used to initialise it
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (depth, plies, tt) -> graph
-    std::map<std::tuple<int, int, int>, int> tuned;          // (depth, size bucket, tt) -> plies
+    std::map<std::tuple<int, int, int>, std::pair<int, bool>> tuned;  // (depth, size bucket, tt) -> plies, tt use
 };
 static BatchState g_batch;
 
@@ -739,38 +740,44 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
     return runGroupSplit(g, roots + h, states + h, nRoots - h, useGraph, counts + h);
 }
 
-static int chooseLeafPlies(int depth, const std::vector<QuadBitBoard> &roots, const std::vector<u8> &states,
-                           bool useGraph, bool useTT)
+// Autotuner: picks a group's leaf plies (1 or 2: where the BFS hands over to
+// the fused leaf) and, when the table is on, whether the group uses it (its
+// probes and stores cost more than they save on shallow groups), by timing
+// every combination on a sample of the group.  Each combination first runs
+// on one sample (graph capture, first launches, table warm-up) and is timed
+// on a second, disjoint one when the group has room, so a timed run does not
+// hit its own entries.  The sample shrinks 4x per ply above depth 4 so that
+// tuning a deep group stays cheap.  Choices are cached per (depth, log2 size,
+// table requested).
+static std::pair<int, bool> chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roots,
+                                            const std::vector<u8> &states, bool useGraph, bool useTT)
 {
     int bucket = 0;
     for (u64 n = roots.size(); n > 1; n >>= 1) bucket++;
     auto key = std::make_tuple(depth, bucket, (int)useTT);
     auto it = g_batch.tuned.find(key);
     if (it != g_batch.tuned.end()) return it->second;
-    // time both leaf forms on a sample of the group (second run of each, so
-    // graph capture and first-launch costs are excluded); the sample shrinks
-    // 4x per ply above depth 4 so tuning a deep group stays cheap
     u64 ns = 256;
     for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
     ns = std::min<u64>(roots.size(), ns);
+    const u64 t0i = roots.size() >= 2 * ns ? ns : 0;  // timed sample [t0i, t0i + ns)
     std::vector<u64> tmp(ns);
     double best = 1e30;
-    int bestP = 2;
-    for (int p = 1; p <= 2 && p < depth; p++) {
-        GroupPlan g{depth, p, depth - p, useTT};
-        double t = 0;
-        for (int rep = 0; rep < 2; rep++) {
-            auto t0 = std::chrono::steady_clock::now();
-            if (runGroupSplit(g, roots.data(), states.data(), ns, useGraph, tmp.data()) != PG_OK) {
-                t = 1e30;
-                break;
+    std::pair<int, bool> choice{2, useTT};
+    for (int tt = useTT ? 1 : 0; tt >= 0; tt--) {
+        for (int p = 1; p <= 2 && p < depth; p++) {
+            GroupPlan g{depth, p, depth - p, tt != 0};
+            double t = 1e30;
+            if (runGroupSplit(g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
+                auto a = std::chrono::steady_clock::now();
+                if (runGroupSplit(g, roots.data() + t0i, states.data() + t0i, ns, useGraph, tmp.data()) == PG_OK)
+                    t = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
             }
-            t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (t < best) best = t, choice = {p, tt != 0};
         }
-        if (t < best) best = t, bestP = p;
     }
-    g_batch.tuned[key] = bestP;
-    return bestP;
+    g_batch.tuned[key] = choice;
+    return choice;
 }
 
 int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_batch_options *opt, uint64_t *counts)
@@ -811,10 +818,15 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
             states[k] = pos[ids[k]].state;
             if (pos[ids[k]].color == BLACK) flipColors(roots[k], states[k]);
         }
-        int plies = forcedPlies == 1 || forcedPlies == 2 ? forcedPlies
-                                                          : chooseLeafPlies(depth, roots, states, useGraph, useTT);
+        int plies = forcedPlies;
+        bool groupTT = useTT;
+        if (plies != 1 && plies != 2) {
+            std::pair<int, bool> c = chooseGroupPlan(depth, roots, states, useGraph, useTT);
+            plies = c.first;
+            groupTT = c.second;
+        }
         if (plies >= depth) plies = depth - 1;
-        GroupPlan g{depth, plies, depth - plies, useTT};
+        GroupPlan g{depth, plies, depth - plies, groupTT};
         std::vector<u64> out(ids.size());
         PG_TRY(runGroupSplit(g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
         for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];

@@ -83,6 +83,19 @@ def test_batch_ep_evasions(gpu, leaf):
         gpu.tt_free()
 
 
+def test_batch_autotuned_with_tt(gpu):
+    # no forced leaf: the autotuner picks leaf plies and table use per depth
+    # group (timed on samples of the group); any choice gives the golden counts
+    rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] <= 5][:1200]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    gpu.tt_init(1024, 0)
+    try:
+        for rep in range(2):
+            assert gpu.perft_batch(ps, [r["depth"] for r in rows], use_tt=True) == [r["count"] for r in rows]
+    finally:
+        gpu.tt_free()
+
+
 def test_batch_graph_replay_and_eager_agree(gpu):
     rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (3, 4)][:600]
     ps = [gpu.parse_fen(r["fen"]) for r in rows]
