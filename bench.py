@@ -292,8 +292,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(a position's tree would be split at the root across GPUs with one 8-byte sum per GPU).",
+        print("bench.py: the engine runs on one GPU and has no multi-GPU mode "
+              "(one would split a position's tree at the root across GPUs, with one 8-byte sum per GPU).",
               file=sys.stderr)
         sys.exit(2)
 
