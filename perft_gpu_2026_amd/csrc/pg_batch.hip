@@ -52,6 +52,9 @@ struct LvlDev {
     u32 *parts;            // expand partitions of the next level
     u64 *known;            // TT mode: table hit + 1 (0 = miss)
     u64 *val;              // TT mode: subtree count of each entry
+    u32 *redir;            // TT mode, levels >= 1: entry an identical position merges into
+    DedupEntry *dtab;      // TT mode, levels >= 1: merge table (dmask + 1 slots, >= 2n)
+    u64 dmask;
 };
 
 struct BatchDev {
@@ -93,40 +96,35 @@ __global__ void k_b_init(BatchDev *d)
     }
 }
 
-// wave-aggregated TT counters
-__device__ __forceinline__ void bttCount(BatchDev *d, bool probed, bool hit, bool stored)
-{
-    u64 pb = __ballot(probed), hb = __ballot(hit), sb = __ballot(stored);
-    if ((threadIdx.x & 63) == 0) {
-        if (pb) atomicAdd((unsigned long long *)&d->ttCtr[0], (unsigned long long)__popcll(pb));
-        if (hb) atomicAdd((unsigned long long *)&d->ttCtr[1], (unsigned long long)__popcll(hb));
-        if (sb) atomicAdd((unsigned long long *)&d->ttCtr[2], (unsigned long long)__popcll(sb));
-    }
-}
-
 __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
 {
     if (d->oom) return;
     const LvlDev a = d->lv[0];
+    const bool useTT = d->useTT;
+    TTEntry *const tt = d->tt;
+    const u64 ttMask = d->ttMask;
+    const int depth = (int)d->depth;
+    TTTally tally;
     u64 tiles = (a.n + BS - 1) / BS;
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u64 i = t * BS + threadIdx.x;
         u32 c = 0;
         bool hit = false;
         if (i < a.n) {
-            if (d->useTT) {
+            if (useTT) {
                 u64 v;
-                hit = ttProbe(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], WHITE), (int)d->depth, &v);
+                hit = ttProbe(tt, ttMask, hashPosition(a.boards[i], a.states[i], WHITE), depth, &v);
                 a.known[i] = hit ? v + 1 : 0;
+                tally.add(true, hit, false);
             }
             c = hit ? 0 : countChildren<WHITE>(a.boards[i], a.states[i]);
             a.cnt[i] = c;
         }
-        if (d->useTT) bttCount(d, i < a.n, hit, false);
         u64 s = blockSum64(c);
         if (threadIdx.x == 0) a.tileSums[t] = (u32)s;
         __syncthreads();
     }
+    if (useTT) ttFlush(d->ttCtr, tally);
 }
 
 template <int C>  // side to move in the parent boards
@@ -136,6 +134,27 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
     const LvlDev a = d->lv[L];
     const LvlDev p = d->lv[L - 1];
     u64 tiles = (a.n + BS - 1) / BS;
+    if (d->useTT) {
+        // make + probe; the counts follow the level's merge (k_b_dedup_count)
+        TTEntry *const tt = d->tt;
+        const u64 ttMask = d->ttMask;
+        const int rd = (int)d->depth - L;
+        TTTally tally;
+        for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
+            u32 pi = a.idx[i];
+            QuadBitBoard q = p.boards[pi];
+            u8 st = p.states[pi];
+            makeMove<C>(q, st, a.mv[i]);
+            a.boards[i] = q;
+            a.states[i] = st;
+            u64 v;
+            bool hit = ttProbe(tt, ttMask, hashPosition(q, st, 1 - C), rd, &v);
+            a.known[i] = hit ? v + 1 : 0;
+            tally.add(true, hit, false);
+        }
+        ttFlush(d->ttCtr, tally);
+        return;
+    }
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u64 i = t * BS + threadIdx.x;
         u32 c = 0;
@@ -146,19 +165,72 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
             makeMove<C>(q, st, a.mv[i]);
             a.boards[i] = q;
             a.states[i] = st;
-            bool hit = false;
-            if (d->useTT) {
-                u64 v;
-                hit = ttProbe(d->tt, d->ttMask, hashPosition(q, st, 1 - C), (int)d->depth - L, &v);
-                a.known[i] = hit ? v + 1 : 0;
-            }
-            c = hit ? 0 : countChildren<1 - C>(q, st);
+            c = countChildren<1 - C>(q, st);
             a.cnt[i] = c;
         }
         u64 s = blockSum64(c);
         if (threadIdx.x == 0) a.tileSums[t] = (u32)s;
         __syncthreads();
     }
+}
+
+// TT mode, levels >= 1: merge identical positions of level L (the scheme of
+// the plain engine's k_dedup_*), then count the children of the entries that
+// remain to be expanded.  The table is cleared by a kernel, not a memset node,
+// so the captured graph replays correctly.
+__global__ __launch_bounds__(BS) void k_b_dedup_clear(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev a = d->lv[L];
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i <= a.dmask; i += (u64)gridDim.x * BS)
+        a.dtab[i] = DedupEntry{0, 0};
+}
+
+template <int C>  // side to move in level L's boards
+__global__ __launch_bounds__(BS) void k_b_dedup_write(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev a = d->lv[L];
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
+        if (a.known[i]) continue;
+        Hash128 h = hashPosition(a.boards[i], a.states[i], C);
+        a.dtab[dedupSlot(h, a.dmask)] = DedupEntry{h.hi, i + 1};
+    }
+}
+
+template <int C>  // side to move in level L's boards
+__global__ __launch_bounds__(BS) void k_b_dedup_count(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev a = d->lv[L];
+    u64 tiles = (a.n + BS - 1) / BS;
+    for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
+        u64 i = t * BS + threadIdx.x;
+        u32 c = 0;
+        if (i < a.n) {
+            u32 r = (u32)i;
+            if (!a.known[i]) {
+                QuadBitBoard q = a.boards[i];
+                u8 st = a.states[i];
+                r = dedupTarget(a.boards, a.states, a.n, a.dtab, a.dmask, i, q, st, hashPosition(q, st, C));
+                if (r == (u32)i) c = countChildren<C>(q, st);
+            }
+            a.redir[i] = r;
+            a.cnt[i] = c;
+        }
+        u64 s = blockSum64(c);
+        if (threadIdx.x == 0) a.tileSums[t] = (u32)s;
+        __syncthreads();
+    }
+}
+
+// duplicates take the value of the entry they merged into
+__global__ __launch_bounds__(BS) void k_b_redirect(BatchDev *d, int L)
+{
+    if (d->oom) return;
+    const LvlDev a = d->lv[L];
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS)
+        if (a.redir[i] != (u32)i) a.val[i] = a.val[a.redir[i]];
 }
 
 __global__ __launch_bounds__(TOP_BS) void k_b_scan_top(BatchDev *d, int L)
@@ -202,7 +274,14 @@ __global__ void k_b_alloc(BatchDev *d, int L)
         b.cnt = (u32 *)dalloc(d, 4 * c + 4);
         b.tileSums = (u32 *)dalloc(d, 4 * bt + 4);
         b.tileOffs = (u64 *)dalloc(d, 8 * (bt + 1));
-        if (d->useTT) b.known = (u64 *)dalloc(d, 8 * c + 8);
+        if (d->useTT) {
+            u64 slots = 256;
+            while (slots < 2 * c) slots <<= 1;
+            b.known = (u64 *)dalloc(d, 8 * c + 8);
+            b.redir = (u32 *)dalloc(d, 4 * c + 4);
+            b.dtab = (DedupEntry *)dalloc(d, sizeof(DedupEntry) * slots);
+            b.dmask = slots - 1;
+        }
     }
     if (d->useTT) b.val = (u64 *)dalloc(d, 8 * c + 8);
     if (!d->oom) b.n = c;
@@ -442,6 +521,13 @@ __global__ __launch_bounds__(BS) void k_b_leaf1v(BatchDev *d)
     }
 }
 
+// TT-mode two-ply leaf with per-entry values and the depth-2 table.  Chunks
+// are taken by a static stride over the grid, not from the queue: the waves
+// in flight then hold chunks spread across the level, whereas in-order
+// chunks put the copies of a position that nearby parents share (A-then-B and
+// B-then-A) in flight together, where none of them finds the other's stored
+// count (a single-root perft 7 made 25 % more leaf misses and took twice as
+// long that way).
 template <int C>
 __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
 {
@@ -454,12 +540,12 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
     const LvlDev a = d->lv[L];
     const LvlDev p = d->lv[L - 1];
     const u64 n = a.n;
+    TTEntry *const tt = d->tt;
+    const u64 ttMask = d->ttMask;
+    TTTally tally;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (;;) {
-        u64 base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long *)&d->work, (unsigned long long)B_CHUNK);
-        base = __shfl(base, 0);
-        if (base >= n) break;
+    const u64 stride = (u64)gridDim.x * BL2_WAVES * B_CHUNK;
+    for (u64 base = ((u64)blockIdx.x * BL2_WAVES + w) * B_CHUNK; base < n; base += stride) {
         u32 cnt = 0;
         u64 e = base + lane, hv = 0;
         bool valid = lane < B_CHUNK && e < n, hit = false;
@@ -470,7 +556,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
             makeMove<C>(q, st, a.mv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            hit = ttProbe(d->tt, d->ttMask, hashPosition(q, st, 1 - C), 2, &hv);
+            hit = ttProbe(tt, ttMask, hashPosition(q, st, 1 - C), 2, &hv);
             cnt = hit ? 0 : countChildren<1 - C>(q, st);
         }
         if (lane < B_CHUNK) sacc[w][lane] = 0;
@@ -505,11 +591,12 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
         if (valid) {
             u64 v = hit ? hv : sacc[w][lane];
             a.val[e] = v;
-            if (store) ttStore(d->tt, d->ttMask, hashPosition(sb[w][lane], ss[w][lane], 1 - C), 2, v);
+            if (store) ttStore(tt, ttMask, hashPosition(sb[w][lane], ss[w][lane], 1 - C), 2, v);
         }
-        bttCount(d, valid, hit, store);
+        tally.add(valid, hit, store);
         waveLdsSync();
     }
+    ttFlush(d->ttCtr, tally);
 }
 
 template <int C>  // side to move in level-L boards
@@ -519,21 +606,24 @@ __global__ __launch_bounds__(BS) void k_b_upsweep(BatchDev *d, int L)
     const LvlDev a = d->lv[L];
     const u64 *cv = d->lv[L + 1].val;
     const int rd = (int)d->depth - L;
+    TTEntry *const tt = d->tt;
+    const u64 ttMask = d->ttMask;
+    TTTally tally;
     for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
+        if (a.redir[i] != (u32)i) continue;  // duplicates are filled in by k_b_redirect
         u64 v;
-        bool store = false;
         if (a.known[i]) {
             v = a.known[i] - 1;
         } else {
             v = 0;
             u64 b = i ? a.cnt[i - 1] : 0, e = a.cnt[i];
             for (u64 k = b; k < e; k++) v += cv[k];
-            ttStore(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], C), rd, v);
-            store = true;
+            ttStore(tt, ttMask, hashPosition(a.boards[i], a.states[i], C), rd, v);
+            tally.add(false, false, true);
         }
         a.val[i] = v;
-        bttCount(d, false, false, store);
     }
+    ttFlush(d->ttCtr, tally);
 }
 
 __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
@@ -541,6 +631,10 @@ __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
     if (d->oom) return;
     const LvlDev a = d->lv[0];
     const u64 *cv = d->lv[1].val;
+    TTEntry *const tt = d->tt;
+    const u64 ttMask = d->ttMask;
+    const int depth = (int)d->depth;
+    u64 *const rootCounts = d->rootCounts;
     for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
         u64 v;
         if (a.known[i]) {
@@ -549,9 +643,9 @@ __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
             v = 0;
             u64 b = i ? a.cnt[i - 1] : 0, e = a.cnt[i];
             for (u64 k = b; k < e; k++) v += cv[k];
-            ttStore(d->tt, d->ttMask, hashPosition(a.boards[i], a.states[i], WHITE), (int)d->depth, v);
+            ttStore(tt, ttMask, hashPosition(a.boards[i], a.states[i], WHITE), depth, v);
         }
-        d->rootCounts[i] = v;
+        rootCounts[i] = v;
     }
 }
 
@@ -588,6 +682,16 @@ static int enqueueGroup(const GroupPlan &g, hipStream_t st)
         if (L > 0) {
             if (side == BLACK) hipLaunchKernelGGL(k_b_makecount<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
             else hipLaunchKernelGGL(k_b_makecount<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+            if (g.useTT) {
+                hipLaunchKernelGGL(k_b_dedup_clear, dim3(G), dim3(BS), 0, st, d, L);
+                if (side == WHITE) {
+                    hipLaunchKernelGGL(k_b_dedup_write<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
+                    hipLaunchKernelGGL(k_b_dedup_count<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
+                } else {
+                    hipLaunchKernelGGL(k_b_dedup_write<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+                    hipLaunchKernelGGL(k_b_dedup_count<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+                }
+            }
         }
         hipLaunchKernelGGL(k_b_scan_top, dim3(1), dim3(TOP_BS), 0, st, d, L);
         hipLaunchKernelGGL(k_b_scan_apply, dim3(G), dim3(BS), 0, st, d, L);
@@ -603,14 +707,13 @@ static int enqueueGroup(const GroupPlan &g, hipStream_t st)
             if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf1v<WHITE>, dim3(G), dim3(BS), 0, st, d);
             else hipLaunchKernelGGL(k_b_leaf1v<BLACK>, dim3(G), dim3(BS), 0, st, d);
         } else {
-            unsigned R = residentGrid((const void *)(pside == WHITE ? k_b_leaf2v<WHITE> : k_b_leaf2v<BLACK>),
-                                      BL2_WAVES * 64);
-            if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2v<WHITE>, dim3(R), dim3(BL2_WAVES * 64), 0, st, d);
-            else hipLaunchKernelGGL(k_b_leaf2v<BLACK>, dim3(R), dim3(BL2_WAVES * 64), 0, st, d);
+            if (pside == WHITE) hipLaunchKernelGGL(k_b_leaf2v<WHITE>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
+            else hipLaunchKernelGGL(k_b_leaf2v<BLACK>, dim3(G), dim3(BL2_WAVES * 64), 0, st, d);
         }
         for (int L = g.leafLevel - 1; L >= 1; L--) {
             if ((L & 1) == WHITE) hipLaunchKernelGGL(k_b_upsweep<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
             else hipLaunchKernelGGL(k_b_upsweep<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
+            hipLaunchKernelGGL(k_b_redirect, dim3(G), dim3(BS), 0, st, d, L);
         }
         hipLaunchKernelGGL(k_b_rootsum, dim3(G), dim3(BS), 0, st, d);
     } else if (g.plies == 1) {

@@ -42,6 +42,25 @@ __device__ __forceinline__ u64 waveInclScan64(u64 v)
     return v;
 }
 
+// Transposition-table counters: each lane tallies its probes, hits and stores
+// in registers, and a wave adds its totals to ctr[0..2] once, when it is
+// done.  Atomics per chunk on one line (in the batched engine, the line that
+// also holds the table pointer every wave reads) serialised the GPU on it and
+// halved the speed of the TT leaf.
+struct TTTally {
+    u32 probes = 0, hits = 0, stores = 0;
+    __device__ void add(bool p, bool h, bool s) { probes += p, hits += h, stores += s; }
+};
+__device__ __forceinline__ void ttFlush(u64 *ctr, const TTTally &t)
+{
+    u64 p = waveSum64(t.probes), h = waveSum64(t.hits), s = waveSum64(t.stores);
+    if ((threadIdx.x & 63) == 0) {
+        if (p) atomicAdd((unsigned long long *)&ctr[0], (unsigned long long)p);
+        if (h) atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)h);
+        if (s) atomicAdd((unsigned long long *)&ctr[2], (unsigned long long)s);
+    }
+}
+
 // block-wide sum (BS threads), result valid in thread 0
 __device__ __forceinline__ u64 blockSum64(u64 v)
 {

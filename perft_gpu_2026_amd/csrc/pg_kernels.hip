@@ -314,17 +314,7 @@ __global__ __launch_bounds__(BS) void k_make_probe(const QuadBitBoard *__restric
     ttCount(ctr, i < n, hit, false);
 }
 
-// Within-level deduplication (write_dedup_table / check_duplicates /
-// apply_redirects, perft_kernels.cu:486-552): every unknown entry writes
-// {hash.hi, index+1} to a cleared table, then reads its slot back; an entry
-// that finds another index with its key AND an identical position (boards
-// compared, so a hash collision or a torn 16-B slot can never merge two
-// different positions) becomes a duplicate with no children, and takes the
-// original's value after the upsweep.
-struct DedupEntry {
-    u64 key, idx;
-};
-
+// Within-level merging (the table and the comparison are in pg_tt.h)
 template <int C>  // side to move in these boards
 __global__ __launch_bounds__(BS) void k_dedup_write(const QuadBitBoard *__restrict__ b, const u8 *__restrict__ s,
                                                     const u64 *__restrict__ known, u64 n, DedupEntry *__restrict__ tab,
@@ -333,7 +323,7 @@ __global__ __launch_bounds__(BS) void k_dedup_write(const QuadBitBoard *__restri
     u64 i = (u64)blockIdx.x * BS + threadIdx.x;
     if (i >= n || known[i]) return;
     Hash128 h = hashPosition(b[i], s[i], C);
-    tab[(h.lo >> 17) & tmask] = DedupEntry{h.hi, i + 1};
+    tab[dedupSlot(h, tmask)] = DedupEntry{h.hi, i + 1};
 }
 
 template <int C>
@@ -350,13 +340,8 @@ __global__ __launch_bounds__(BS) void k_dedup_count(const QuadBitBoard *__restri
         if (!known[i]) {
             QuadBitBoard q = b[i];
             u8 st = s[i];
-            Hash128 h = hashPosition(q, st, C);
-            DedupEntry e = tab[(h.lo >> 17) & tmask];
-            u64 j = e.idx - 1;
-            bool dup = e.key == h.hi && e.idx && j != i && j < n && s[j] == st && b[j].bb[0] == q.bb[0] &&
-                       b[j].bb[1] == q.bb[1] && b[j].bb[2] == q.bb[2] && b[j].bb[3] == q.bb[3];
-            if (dup) r = (u32)j;
-            else c = countChildren<C>(q, st);
+            r = dedupTarget(b, s, n, tab, tmask, i, q, st, hashPosition(q, st, C));
+            if (r == (u32)i) c = countChildren<C>(q, st);
         }
         redir[i] = r;
         cnt[i] = c;
@@ -388,10 +373,12 @@ __global__ __launch_bounds__(BS) void k_leaf1_val(const QuadBitBoard *__restrict
 
 // two-ply leaf with per-entry values and the depth-2 table (the reference's
 // leaf TT, perft_kernels.cu:867-877): wave-cooperative like k_leaf2, with each
-// entry's partial sum kept in LDS.  A static stride over a grid of up to
-// 8 blocks per CU: serving runs this leaf from several slots at once, where
-// small grids of concurrent requests share the GPU better than a resident
-// grid per request (the chunk-queue form measured 9 % fewer requests/s).
+// entry's partial sum kept in LDS.  Chunks are taken by a static stride over
+// a grid of up to 8 blocks per CU, not from a queue: the waves in flight then
+// hold chunks spread across the level, whereas in-order chunks put the copies
+// of a position that nearby parents share (A-then-B and B-then-A) in flight
+// together, where none of them finds the other's stored count (the queue form
+// served 9 % fewer requests/s).
 template <int C>
 __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard *__restrict__ pb,
                                                              const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
@@ -403,6 +390,7 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard 
     __shared__ u32 sacc[L2_WAVES][L2_CHUNK];
     __shared__ u32 sl[L2_WAVES][L2_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    TTTally tally;
     const u64 stride = (u64)gridDim.x * L2_WAVES * L2_CHUNK;
     for (u64 base = ((u64)blockIdx.x * L2_WAVES + w) * L2_CHUNK; base < n; base += stride) {
         u32 cnt = 0;
@@ -455,9 +443,10 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard 
                 store = true;
             }
         }
-        if (tt) ttCount(ctr, valid, hit, store);
+        tally.add(valid && tt, hit, store);
         waveLdsSync();
     }
+    if (tt) ttFlush(ctr, tally);
 }
 
 // upsweep of one level: value = table hit, or the sum of the children's

@@ -88,6 +88,33 @@ __device__ __forceinline__ void ttStore(TTEntry *tt, u64 mask, const Hash128 &h,
     if (own0 || (u64)depth >= d0) b[0] = e;  // one 16-B store
     else b[1] = e;
 }
+
+// ----------------------------------------------------- within-level merging
+// (write_dedup_table / check_duplicates / apply_redirects,
+// perft_kernels.cu:486-552): every unknown entry of a level writes
+// {hash.hi, index + 1} to a cleared table of at least 2n slots, then reads its
+// slot back; an entry that finds another index with its key AND an identical
+// position (boards compared, so a hash collision or a torn 16-B slot can
+// never merge two different positions) becomes a duplicate with no children,
+// and takes the original's value after the upsweep.
+struct DedupEntry {
+    u64 key, idx;
+};
+
+__device__ __forceinline__ u64 dedupSlot(const Hash128 &h, u64 tmask) { return (h.lo >> 17) & tmask; }
+
+// the entry of the level that position (q, st) at index i merges into: i
+// itself unless the table names an identical other entry
+__device__ __forceinline__ u32 dedupTarget(const QuadBitBoard *__restrict__ b, const u8 *__restrict__ s, u64 n,
+                                           const DedupEntry *__restrict__ tab, u64 tmask, u64 i,
+                                           const QuadBitBoard &q, u8 st, const Hash128 &h)
+{
+    DedupEntry e = tab[dedupSlot(h, tmask)];
+    u64 j = e.idx - 1;
+    bool dup = e.key == h.hi && e.idx && j != i && j < n && s[j] == st && b[j].bb[0] == q.bb[0] &&
+               b[j].bb[1] == q.bb[1] && b[j].bb[2] == q.bb[2] && b[j].bb[3] == q.bb[3];
+    return dup ? (u32)j : (u32)i;
+}
 #endif
 
 // ------------------------------------------------------------------ host TT
