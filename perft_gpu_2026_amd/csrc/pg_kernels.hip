@@ -279,18 +279,8 @@ int g_leafBlocksPerCU = 8;  // grid of the grid-stride leaves, per CU (x2 for th
 // (reference: makemove_and_count_moves_kernel<useTT>, reduce_by_parent,
 // add_tt_hits_and_store, perft_kernels.cu:201-257, 554-599).
 
-// wave-aggregated probe/hit/store counters
-__device__ __forceinline__ void ttCount(u64 *ctr, bool probed, bool hit, bool stored)
-{
-    u64 pb = __ballot(probed), hb = __ballot(hit), sb = __ballot(stored);
-    if ((threadIdx.x & 63) == 0) {
-        if (pb) atomicAdd((unsigned long long *)&ctr[0], (unsigned long long)__popcll(pb));
-        if (hb) atomicAdd((unsigned long long *)&ctr[1], (unsigned long long)__popcll(hb));
-        if (sb) atomicAdd((unsigned long long *)&ctr[2], (unsigned long long)__popcll(sb));
-    }
-}
-
-// make + table probe (TT mode): a hit is a known subtree
+// make + table probe (TT mode): a hit is a known subtree.  Grid-stride over a
+// capped grid, so that each wave adds its counter tallies once.
 template <int C>
 __global__ __launch_bounds__(BS) void k_make_probe(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
                                                    const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
@@ -298,9 +288,8 @@ __global__ __launch_bounds__(BS) void k_make_probe(const QuadBitBoard *__restric
                                                    u64 *__restrict__ known, const TTEntry *__restrict__ tt, u64 mask,
                                                    int rd, u64 *ctr)
 {
-    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
-    bool hit = false;
-    if (i < n) {
+    TTTally tally;
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < n; i += (u64)gridDim.x * BS) {
         u32 p = pidx[i];
         QuadBitBoard q = pb[p];
         u8 st = ps[p];
@@ -308,10 +297,11 @@ __global__ __launch_bounds__(BS) void k_make_probe(const QuadBitBoard *__restric
         ob[i] = q;
         os[i] = st;
         u64 v;
-        hit = ttProbe(tt, mask, hashPosition(q, st, 1 - C), rd, &v);
+        bool hit = ttProbe(tt, mask, hashPosition(q, st, 1 - C), rd, &v);
         known[i] = hit ? v + 1 : 0;
+        tally.add(true, hit, false);
     }
-    ttCount(ctr, i < n, hit, false);
+    ttFlush(ctr, tally);
 }
 
 // Within-level merging (the table and the comparison are in pg_tt.h)
@@ -458,9 +448,9 @@ __global__ __launch_bounds__(BS) void k_upsweep(const QuadBitBoard *__restrict__
                                                 u64 n, u64 *__restrict__ val, TTEntry *__restrict__ tt, u64 mask,
                                                 int rd, u64 *ctr)
 {
-    u64 i = (u64)blockIdx.x * BS + threadIdx.x;
-    bool store = false;
-    if (i < n && redir[i] == (u32)i) {  // duplicates are filled in afterwards
+    TTTally tally;
+    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < n; i += (u64)gridDim.x * BS) {
+        if (redir[i] != (u32)i) continue;  // duplicates are filled in afterwards
         u64 v;
         if (known[i]) {
             v = known[i] - 1;
@@ -470,12 +460,12 @@ __global__ __launch_bounds__(BS) void k_upsweep(const QuadBitBoard *__restrict__
             for (u64 k = b; k < e; k++) v += childVal[k];
             if (tt) {
                 ttStore(tt, mask, hashPosition(boards[i], states[i], C), rd, v);
-                store = true;
+                tally.add(false, false, true);
             }
         }
         val[i] = v;
     }
-    if (tt) ttCount(ctr, false, false, store);
+    if (tt) ttFlush(ctr, tally);
 }
 
 __global__ __launch_bounds__(BS) void k_sum_u64(const u64 *__restrict__ v, u64 n, u64 *__restrict__ sum)
@@ -500,6 +490,15 @@ unsigned residentGrid(const void *fn, int blockSize)
     unsigned grid = (unsigned)perCU * (unsigned)(lib().cuCount ? lib().cuCount : 256);
     cache.emplace(fn, grid);
     return grid;
+}
+
+// grid of the grid-stride TT kernels: at most 8 blocks per CU, so that each
+// wave's counter tallies are added once for many entries
+static unsigned tallyGrid(u64 n)
+{
+    u64 cap = (u64)(lib().cuCount ? lib().cuCount : 256) * 8;
+    u64 g = (n + BS - 1) / BS;
+    return (unsigned)(g < cap ? (g ? g : 1) : cap);
 }
 
 static int launchMakecount(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, const u16 *mv, int color, u64 n,
@@ -646,16 +645,16 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         while (tslots < 2 * n) tslots <<= 1;
         DedupEntry *tab = A.alloc<DedupEntry>(tslots);
         if (!ob || !os || !cnt || !known || !val || !tileSums || !tileOffs || !redir || !tab) return PG_ERR_ARENA_OOM;
-        dim3 g(gridFor(n, BS)), b(BS);
+        dim3 g(gridFor(n, BS)), gc(tallyGrid(n)), b(BS);
         int rd = depth - lvl;
         PG_HIP(hipMemsetAsync(tab, 0, tslots * sizeof(DedupEntry), st));
         if (pc == WHITE) {
-            hipLaunchKernelGGL(k_make_probe<WHITE>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
+            hipLaunchKernelGGL(k_make_probe<WHITE>, gc, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
             hipLaunchKernelGGL(k_dedup_write<BLACK>, g, b, 0, st, ob, os, known, n, tab, tslots - 1);
             hipLaunchKernelGGL(k_dedup_count<BLACK>, g, b, 0, st, ob, os, known, n, tab, tslots - 1, redir, cnt,
                                tileSums);
         } else {
-            hipLaunchKernelGGL(k_make_probe<BLACK>, g, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
+            hipLaunchKernelGGL(k_make_probe<BLACK>, gc, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
             hipLaunchKernelGGL(k_dedup_write<WHITE>, g, b, 0, st, ob, os, known, n, tab, tslots - 1);
             hipLaunchKernelGGL(k_dedup_count<WHITE>, g, b, 0, st, ob, os, known, n, tab, tslots - 1, redir, cnt,
                                tileSums);
@@ -710,12 +709,12 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
     for (int lvl = (int)lv.size(); lvl >= 1; lvl--) {
         TTLevel &l = lv[lvl - 1];
         int side = color ^ (lvl & 1);
-        dim3 g(gridFor(l.n, BS)), b(BS);
+        dim3 g(gridFor(l.n, BS)), gc(tallyGrid(l.n)), b(BS);
         if (side == WHITE)
-            hipLaunchKernelGGL(k_upsweep<WHITE>, g, b, 0, st, l.boards, l.states, l.incl, l.known, l.redir, childVal,
+            hipLaunchKernelGGL(k_upsweep<WHITE>, gc, b, 0, st, l.boards, l.states, l.incl, l.known, l.redir, childVal,
                                l.n, l.val, tt, mask, depth - lvl, ctr);
         else
-            hipLaunchKernelGGL(k_upsweep<BLACK>, g, b, 0, st, l.boards, l.states, l.incl, l.known, l.redir, childVal,
+            hipLaunchKernelGGL(k_upsweep<BLACK>, gc, b, 0, st, l.boards, l.states, l.incl, l.known, l.redir, childVal,
                                l.n, l.val, tt, mask, depth - lvl, ctr);
         hipLaunchKernelGGL(k_apply_redirects, g, b, 0, st, l.redir, l.n, l.val);
         childVal = l.val;
