@@ -150,14 +150,14 @@ REF_RUNS_TT = [("startpos", 9, "0.174 s on RTX PRO 6000"), ("startpos", 10, "2.0
                ("startpos", 11, "24.33 s on RTX PRO 6000")]
 
 
-def ref_positions_leg(pg, tt_max_depth: int) -> dict:
+def ref_positions_leg(pg, tt_max_depth: int, tt_only: bool = False) -> dict:
     """The reference README's published positions through the launcher, each
     count checked against the published value: without the tables, then
     startpos with them (64 GB device table, 32 GB host tier, cleared before
     each run)."""
     known = {p["name"]: p for p in json.load(open(os.path.join(ROOT, "tests", "golden", "known_positions.json")))["positions"]}
     out = {}
-    for name, d, theirs in REF_RUNS:
+    for name, d, theirs in ([] if tt_only else REF_RUNS):
         k = known[name]
         want = k["counts"][d - 1]
         p = pg.parse_fen(k["fen"])
@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched-vs-plain leg")
     ap.add_argument("--no-tt", action="store_true", help="skip the config-4 transposition-table leg")
     ap.add_argument("--no-ref-positions", action="store_true", help="skip the reference README positions")
+    ap.add_argument("--ref-tt-only", action="store_true",
+                    help="reference-positions leg: only the startpos runs with the tables")
     ap.add_argument("--ref-tt-depth", type=int, default=11,
                     help="deepest startpos perft run with the tables in the reference-positions leg (8 = none)")
     ap.add_argument("--serve", type=int, default=10000, help="config-5 requests to replay (0: skip)")
@@ -365,7 +367,7 @@ def main():
         out["tt_config4"] = tt_leg(pg)
     if not a.no_ref_positions:
         print("[bench] reference README positions", file=sys.stderr, flush=True)
-        out["reference_positions"] = ref_positions_leg(pg, a.ref_tt_depth)
+        out["reference_positions"] = ref_positions_leg(pg, a.ref_tt_depth, a.ref_tt_only)
     if a.serve:
         print("[bench] config-5 serving", file=sys.stderr, flush=True)
         out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate, a.serve_tt_gb)
