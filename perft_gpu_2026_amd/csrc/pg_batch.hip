@@ -522,12 +522,12 @@ __global__ __launch_bounds__(BS) void k_b_leaf1v(BatchDev *d)
 }
 
 // TT-mode two-ply leaf with per-entry values and the depth-2 table.  Chunks
-// are taken by a static stride over the grid, not from the queue: the waves
-// in flight then hold chunks spread across the level, whereas in-order
-// chunks put the copies of a position that nearby parents share (A-then-B and
-// B-then-A) in flight together, where none of them finds the other's stored
-// count (a single-root perft 7 made 25 % more leaf misses and took twice as
-// long that way).
+// are visited in a fixed permuted order rather than taken in order from the
+// queue: the chunks in flight together are then spread over the level,
+// whereas in-order chunks put the copies of a position that nearby parents
+// share (A-then-B and B-then-A) in flight together, where none of them finds
+// the other's stored count (a single-root perft 7 made 25 % more leaf misses
+// in order).
 template <int C>
 __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
 {
@@ -544,8 +544,13 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
     const u64 ttMask = d->ttMask;
     TTTally tally;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const u64 stride = (u64)gridDim.x * BL2_WAVES * B_CHUNK;
-    for (u64 base = ((u64)blockIdx.x * BL2_WAVES + w) * B_CHUNK; base < n; base += stride) {
+    // chunk k of each wave's sequence is chunk k * P mod nChunks of the level
+    // (P a prime above any chunk count): the chunks in flight together are
+    // spread over the whole level
+    const u64 nChunks = (n + B_CHUNK - 1) / B_CHUNK;
+    const u64 P = 1000000007ULL % (nChunks ? nChunks : 1);
+    for (u64 k = (u64)blockIdx.x * BL2_WAVES + w; k < nChunks; k += (u64)gridDim.x * BL2_WAVES) {
+        const u64 base = ((k * P) % nChunks) * B_CHUNK;
         u32 cnt = 0;
         u64 e = base + lane, hv = 0;
         bool valid = lane < B_CHUNK && e < n, hit = false;
