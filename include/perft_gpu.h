@@ -80,6 +80,10 @@ int pg_parse_fen(const char *fen, pg_position *pos);
 int pg_write_fen(const pg_position *pos, char *out, int outLen);
 /* legal moves of a position, host side (generateMoves, MoveGeneratorBitboard.h:2072) */
 int pg_gen_moves(const pg_position *pos, uint16_t *moves, int *n);
+/* perft(1) as the reference counts it (countMoves, MoveGeneratorBitboard.h:
+ * 1563-1943): the number of legal moves, except that a side in check from a
+ * pawn that has just double-pushed counts its en-passant captures once even
+ * when two pawns can take (:1639-1649) */
 int pg_count_moves(const pg_position *pos, int *n);
 int pg_make_move(pg_position *pos, uint16_t move);
 /* seeded random walk used by the workloads (no reference counterpart): SplitMix64
