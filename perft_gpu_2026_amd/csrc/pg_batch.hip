@@ -322,8 +322,19 @@ __global__ __launch_bounds__(BS) void k_b_expand(BatchDev *d, int L)
         expandTileParents(a.cnt, a.parts[t], a.parts[t + 1], c0, tile, wmax, o);
         // each child's root: the parent's root (level 1 entries are roots' children)
         const u64 base = c0 + (u64)threadIdx.x * EXPAND_PER;
+        if (L == 0) {
 #pragma unroll
-        for (int k = 0; k < EXPAND_PER; k++) r[k] = (L == 0 || base + k >= c) ? o[k] : a.root[o[k]];
+            for (int k = 0; k < EXPAND_PER; k++) r[k] = o[k];
+        } else {
+            // (a select between o[k] and a.root[o[k]] made the compiler spill
+            // o to scratch and load through a selected pointer)
+#pragma unroll
+            for (int k = 0; k < EXPAND_PER; k++) {
+                u32 x = 0;
+                if (base + k < c) x = a.root[o[k]];
+                r[k] = x;
+            }
+        }
         storeTileStreaming(tile, o, b.idx, c0, c);
         __syncthreads();
         storeTileStreaming(tile, r, b.root, c0, c);
