@@ -59,6 +59,19 @@ def test_tt_tiny_table_still_exact(gpu):
         gpu.tt_free()
 
 
+def test_tt_oversized_request_is_halved(gpu, capfd):
+    # 512 GB does not fit beside the arena: the table halves until it does
+    gpu.tt_init(512 * 1024, 0)
+    try:
+        assert "does not fit" in capfd.readouterr().err
+        p = gpu.parse_fen(KIWI)
+        before = gpu.tt_stats()
+        assert gpu.perft_bfs(p, 5, use_tt=True) == 193690690
+        assert gpu.tt_stats()[0] > before[0]  # the device table exists and was probed
+    finally:
+        gpu.tt_free()
+
+
 @pytest.mark.parametrize("leaf", [1, 2])
 def test_batch_matches_golden_c3(gpu, leaf):
     rows = load_jsonl("c3.jsonl")
