@@ -18,6 +18,8 @@ The JSON line carries:
                 P parent boards).  The leaf is VALU-bound integer work, so
                 its HBM fraction is small by nature; `roofline_levels` gives
                 the same for the BFS level kernels, which are HBM-bound.
+  valu_issue    the leaf's VALU issue rate per SIMD and cycle: its measured
+                instruction count per launch over the live launch time.
   cpu_baseline  the oracle (a C restatement of the reference's single-threaded
                 -cpu path) on this host, one core, on a bounded sample of the
                 same tree: perft(6) under the first root moves.
@@ -74,7 +76,27 @@ def cpu_baseline(fen: str, depth: int, budget_s: float = 15.0) -> dict:
 # kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0);
 # the writes are mostly the chunk queue's atomics.
 # Commands and figures: profiles/leaf2_config2_pmc.md.
-PMC_TRAFFIC = {"leaf2": 66232.0 * 1024 * 2.0 + 7586.3 * 1024 * 1.0}
+PMC_TRAFFIC = {"leaf2": 66224.0 * 1024 * 2.0 + 7586.3 * 1024 * 1.0}
+
+# VALU work per launch of the same leaf: SQ_INSTS_VALU (wave64 instructions)
+# and SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64), the share of lanes
+# active per VALU instruction, from a third rocprofv3 --pmc run of this bench
+# (profiles/leaf2_config2_pmc.md).  Divided by the live launch time, the
+# instruction count gives the issue rate per SIMD and cycle (1,024 SIMDs at
+# the 2.4 GHz peak clock); one wave64 VALU instruction per 4 cycles per SIMD
+# is 0.25.
+PMC_VALU = {"leaf2": (4.6467e9, 0.91)}
+SIMDS, CLOCK_HZ = 1024, 2.4e9
+
+
+def valu_issue(stat: dict):
+    if stat["name"] not in PMC_VALU:
+        return None
+    insts, lanes = PMC_VALU[stat["name"]]
+    avg_s = stat["ms"] / stat["launches"] * 1e-3
+    return {"kernel": stat["name"], "wave_insts_per_launch": insts, "lanes_active": lanes,
+            "wave_insts_per_simd_cycle": round(insts / (SIMDS * CLOCK_HZ * avg_s), 4),
+            "one_per_4_cycles": 0.25}
 
 
 def roofline(stat: dict, traffic=None) -> dict:
@@ -355,6 +377,7 @@ def main():
                    "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
         "roofline": roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None) if leaf else None,
         "roofline_levels": roofline(top_level) if top_level else None,
+        "valu_issue": valu_issue(leaf) if leaf and a.leaf2_variant == 0 else None,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
                     for k, v in stats.items()},
