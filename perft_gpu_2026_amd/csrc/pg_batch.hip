@@ -15,10 +15,14 @@
 //    reads its size from the level descriptor.  With fixed launch shapes the
 //    whole level sequence of a group is captured once in a hipGraph and
 //    replayed for every batch of the same shape;
-//  * the leaf is persistent: waves pull 32-entry chunks from a device work
+//  * the leaf is persistent: waves pull 64-entry chunks from a device work
 //    queue (an atomic cursor), so roots with huge subtrees and roots with tiny
 //    ones share the GPU without a tail; results go to per-root u64 counters
 //    with one atomic per wave and root;
+//  * with the transposition table every level after the roots probes it,
+//    merges identical positions across all roots and counts only what is left
+//    to expand; the TT leaf visits its chunks in a fixed permuted order and an
+//    upsweep stores every merged subtree once;
 //  * an autotuner picks the leaf plies (1 or 2) and, with the table on,
 //    whether a group uses it, per (depth, group size), by timing each choice
 //    on a sample of the group, and caches the choice.
