@@ -872,6 +872,7 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
 // hit its own entries.  The sample shrinks 4x per ply above depth 4 so that
 // tuning a deep group stays cheap.  Choices are cached per (depth, log2 size,
 // table requested).
+constexpr int TUNE_MAX_DEPTH = 7;  // groups this deep are not tuned
 static std::pair<int, bool> chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roots,
                                             const std::vector<u8> &states, bool useGraph, bool useTT)
 {
@@ -880,6 +881,10 @@ static std::pair<int, bool> chooseGroupPlan(int depth, const std::vector<QuadBit
     auto key = std::make_tuple(depth, bucket, (int)useTT);
     auto it = g_batch.tuned.find(key);
     if (it != g_batch.tuned.end()) return it->second;
+    // Deep groups are leaf-bound, where the fused two-ply leaf always wins
+    // and the table always pays; a one-ply trial there would cost more than
+    // a whole shallow batch (startpos children at depth 9: minutes).
+    if (depth >= TUNE_MAX_DEPTH) return g_batch.tuned[key] = {2, useTT};
     u64 ns = 256;
     for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
     ns = std::min<u64>(roots.size(), ns);

@@ -109,6 +109,16 @@ def test_batch_autotuned_with_tt(gpu):
         gpu.tt_free()
 
 
+def test_batch_deep_group_not_tuned(gpu):
+    # depth >= 7: the tuner picks the two-ply leaf without trials (a one-ply
+    # trial on such roots costs minutes); counts as golden
+    r = load_jsonl("c2.jsonl")[0]
+    p = gpu.parse_fen(r["fen"])
+    t0 = time.perf_counter()
+    assert gpu.perft_batch([p, p], [7, 7]) == [r["counts"][6]] * 2
+    assert time.perf_counter() - t0 < 30.0
+
+
 def test_batch_graph_replay_and_eager_agree(gpu):
     rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (3, 4)][:600]
     ps = [gpu.parse_fen(r["fen"]) for r in rows]
