@@ -17,7 +17,10 @@ The JSON line carries:
                 and the algorithmic bytes 6K + 33P per launch (K leaf entries,
                 P parent boards).  The leaf is VALU-bound integer work, so
                 its HBM fraction is small by nature; `roofline_levels` gives
-                the same for the BFS level kernels, which are HBM-bound.
+                the same for the BFS level kernels, which are HBM-bound, at
+                config 2's own level sizes (small: launch-latency bound), and
+                `roofline_levels_full_size` gives scan and expand on the
+                largest level of the tree (config 2 run with the one-ply leaf).
   valu_issue    the leaf's VALU issue rate per SIMD and cycle: its measured
                 instruction count per launch over the live launch time.
   cpu_baseline  the oracle (a C restatement of the reference's single-threaded
@@ -107,6 +110,42 @@ def roofline(stat: dict, traffic=None) -> dict:
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": stat["name"], "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+
+
+def levels_leg(pg, pos, depth: int, want: int, leaf_plies: int, steps: int = 3) -> dict:
+    """The BFS level kernels at full size: with the one-ply leaf the config-2
+    BFS builds its perft(6) level itself (15,143,580 boards, 414,873,802
+    children), so each level kernel's largest launch runs on the largest level
+    this tree has.  Per kernel: the mean HIP-event time of those launches (one
+    per step) and their algorithmic bytes.  scan (8N) and expand (4N + 4C) are
+    HBM-bound; makecount and genmoves are VALU-bound and listed for scale."""
+    pg.set_leaf_plies(1)
+    try:
+        assert pg.perft_bfs(pos, depth) == want
+        pg.profile_reset()
+        pg.profile_enable(True)
+        for _ in range(steps):
+            c = pg.perft_bfs(pos, depth)
+            if c != want:
+                raise SystemExit(f"count mismatch in the level leg: {c} != {want}")
+        pg.device_synchronize()
+        pg.profile_enable(False)
+        stats = pg.profile_get()
+    finally:
+        pg.set_leaf_plies(leaf_plies)
+    out = {"workload": "config-2 perft 7 with the one-ply leaf: largest level 15,143,580 boards -> 414,873,802 "
+                       "children; the largest launch of each kernel, one per step",
+           "steps": steps}
+    for k in ("scan", "expand", "makecount", "genmoves"):
+        v = stats.get(k)
+        if not v or not v["big_ms"]:
+            continue
+        gbs = v["big_bytes"] / (v["big_ms"] * 1e-3) / 1e9
+        out[k] = {"bound": "hbm" if k in ("scan", "expand") else "valu", "achieved": round(gbs, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                  "avg_launch_ms": round(v["big_ms"], 4), "launches": v["big_launches"],
+                  "algorithmic_bytes_per_launch": round(v["big_bytes"])}
+    return out
 
 
 def batch_leg(pg) -> dict:
@@ -302,6 +341,7 @@ def main():
     ap.add_argument("--leaf2-variant", type=int, default=0)
     ap.add_argument("--leaf-grid", type=int, default=0, help="leaf blocks per CU (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-levels", action="store_true", help="skip the full-size level-kernel leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched-vs-plain leg")
     ap.add_argument("--no-tt", action="store_true", help="skip the config-4 transposition-table leg")
     ap.add_argument("--no-ref-positions", action="store_true", help="skip the reference README positions")
@@ -383,6 +423,9 @@ def main():
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
                     for k, v in stats.items()},
     }
+    if not a.no_levels:
+        print("[bench] level kernels at full size", file=sys.stderr, flush=True)
+        out["roofline_levels_full_size"] = levels_leg(pg, pos, depth, want, a.leaf or 2)
     if not a.no_batch:
         print("[bench] config-3 batch", file=sys.stderr, flush=True)
         out["batch_config3"] = batch_leg(pg)

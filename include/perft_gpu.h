@@ -168,7 +168,7 @@ int pg_level_makecount(const void *parentBoards, const uint8_t *parentStates, co
 /* cub::DeviceScan::InclusiveSum replacement, perft_kernels.cu:786: in-place
  * inclusive scan of n u32 with a wave64 block scan.  tileSums[t] is the sum
  * of data[256t .. 256t+255]; if it is NULL the sums are computed here.  scratch
- * must hold 8*(ceil(n/256)+1) bytes, plus 4*ceil(n/256) more when tileSums is
+ * must hold 8*(ceil(n/256)+4) bytes, plus 4*ceil(n/256) more when tileSums is
  * NULL; *total (host pointer, may be NULL) receives the 64-bit sum
  * (synchronises the stream). */
 int pg_scan_u32(uint32_t *data, uint64_t n, const uint32_t *tileSums, void *scratch, uint64_t *total, void *stream);
@@ -194,6 +194,11 @@ typedef struct pg_kernel_stat {
     double   ms;      /* summed HIP-event time of the launches */
     double   bytes;   /* summed algorithmic bytes of the launches */
     double   items;   /* summed entries processed */
+    /* the launches with the most algorithmic bytes (a repeated call's largest
+     * level): their bytes each, their mean time and their number */
+    double   big_bytes;
+    double   big_ms;
+    uint64_t big_launches;
 } pg_kernel_stat;
 int pg_profile_enable(int on);
 int pg_profile_reset(void);

@@ -65,7 +65,7 @@ class ServeResult(ctypes.Structure):
 
 class KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", c_uint64), ("ms", c_double), ("bytes", c_double),
-                ("items", c_double)]
+                ("items", c_double), ("big_bytes", c_double), ("big_ms", c_double), ("big_launches", c_uint64)]
 
 
 class PerftError(RuntimeError):
@@ -323,7 +323,8 @@ def profile_get() -> dict[str, dict]:
     n = c_int()
     _check(lib().pg_profile_get(arr, 16, byref(n)), "pg_profile_get")
     return {arr[i].name.decode(): {"launches": arr[i].launches, "ms": arr[i].ms, "bytes": arr[i].bytes,
-                                   "items": arr[i].items} for i in range(n.value)}
+                                   "items": arr[i].items, "big_bytes": arr[i].big_bytes, "big_ms": arr[i].big_ms,
+                                   "big_launches": arr[i].big_launches} for i in range(n.value)}
 
 
 def device_synchronize() -> None:

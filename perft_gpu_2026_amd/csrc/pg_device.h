@@ -80,10 +80,14 @@ __device__ __forceinline__ u64 blockSum64(u64 v)
 // each wave load covers 1 KB of consecutive sums; a wave scan of the thread
 // totals and the waves' totals in LDS give every sum its offset.  TOP_ROWS
 // rows share one barrier and have their loads in flight together.
+// ACC is the type the waves' totals of a row are added in: u32 for tile sums
+// (a row of 4,096 at most 256 * 218 each), u64 for group sums (64 tiles each).
 constexpr int TOP_BS = 1024;
 constexpr int TOP_ROWS = 8;
+template <typename ACC = u32, int ROWS = TOP_ROWS>
 __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u64 tiles, u64 *__restrict__ tileOffs)
 {
+    constexpr int TOP_ROWS = ROWS;
     __shared__ u32 wtot[TOP_ROWS][TOP_BS / 64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     constexpr u64 ROW = 4ull * TOP_BS, PASS = ROW * TOP_ROWS;
@@ -108,7 +112,8 @@ __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u
                 }
             }
         }
-        // a row holds at most 4096 tile sums of at most 256 * 218 each: u32
+        // a thread's four sums and a wave's 256 fit u32 for tile sums (at
+        // most 256 * 218) and group sums (at most 64 tiles of those)
 #pragma unroll
         for (int r = 0; r < TOP_ROWS; r++) {
             if (r < rows) {
@@ -123,10 +128,10 @@ __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u
 #pragma unroll
         for (int r = 0; r < TOP_ROWS; r++) {
             if (r < rows) {
-                u32 before = 0, all = 0;
+                ACC before = 0, all = 0;
 #pragma unroll
                 for (int k = 0; k < TOP_BS / 64; k++) {
-                    u32 t = wtot[r][k];
+                    ACC t = wtot[r][k];
                     all += t;
                     before += k < w ? t : 0u;
                 }
@@ -144,6 +149,113 @@ __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u
         __syncthreads();  // wtot is rewritten by the next pass
     }
     if (tid == 0) tileOffs[tiles] = carry;
+}
+
+// ------------------------------------------------- grouped scan (plain BFS)
+// The tile sums are scanned in groups of SCAN_GROUP tiles, in three passes:
+// scanGroups (all CUs) writes every tile's offset inside its group and the
+// group sums; one block scans the group sums (scanTileSums over them); each
+// apply wave adds its group's offset and its in-group offset.  A single-block
+// pass over the tile sums themselves was store-bound on one CU (26 us for the
+// 59,155 tiles of a 15-million-entry level with per-tile 64-bit offsets,
+// 12.7 us with per-group offsets and a wave sum in the apply).
+constexpr int SCAN_GROUP = 64;
+__host__ __device__ constexpr u64 scanGroupCount(u64 tiles) { return (tiles + SCAN_GROUP - 1) / SCAN_GROUP; }
+
+// One wave per four groups: lane l holds tiles 4l..4l+3 of the wave's 256
+// (one 16-byte load when whole and aligned), so a group is one 16-lane row;
+// a row scan gives the in-group offsets (at most 64 * 256 * 218: u32).
+__device__ __forceinline__ void scanGroups(const u32 *__restrict__ tileSums, u64 tiles, u32 *__restrict__ tileIn,
+                                           u32 *__restrict__ groupSums, u64 quad)
+{
+    const int lane = threadIdx.x & 63;
+    const u64 t0 = quad * 4 * SCAN_GROUP;
+    if (t0 >= tiles) return;
+    const u64 t = t0 + 4ull * lane;
+    u32 x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    if (t + 4 <= tiles && ((uintptr_t)tileSums & 15) == 0) {
+        uint4 q = *(const uint4 *)(tileSums + t);
+        x0 = q.x, x1 = q.y, x2 = q.z, x3 = q.w;
+    } else {
+        if (t < tiles) x0 = tileSums[t];
+        if (t + 1 < tiles) x1 = tileSums[t + 1];
+        if (t + 2 < tiles) x2 = tileSums[t + 2];
+        if (t + 3 < tiles) x3 = tileSums[t + 3];
+    }
+    const u32 sum = x0 + x1 + x2 + x3;
+    u32 inc = sum;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const u32 v = __shfl_up(inc, o, 16);
+        if ((lane & 15) >= o) inc += v;
+    }
+    const u32 e0 = inc - sum, e1 = e0 + x0, e2 = e1 + x1, e3 = e2 + x2;
+    if (t + 4 <= tiles && ((uintptr_t)tileIn & 15) == 0) {
+        *(uint4 *)(tileIn + t) = make_uint4(e0, e1, e2, e3);
+    } else {
+        if (t < tiles) tileIn[t] = e0;
+        if (t + 1 < tiles) tileIn[t + 1] = e1;
+        if (t + 2 < tiles) tileIn[t + 2] = e2;
+        if (t + 3 < tiles) tileIn[t + 3] = e3;
+    }
+    const u64 g = quad * 4 + (lane >> 4);
+    if ((lane & 15) == 15 && g * SCAN_GROUP < tiles) groupSums[g] = inc;
+}
+
+// The expand partitions from the scan itself: parent p is the first parent of
+// expand tile t iff excl(p) <= t * EXPAND_TILE < incl(p) (a parent has fewer
+// than EXPAND_TILE children, so it starts at most one tile), which is
+// upper_bound(incl, t * EXPAND_TILE); the entry n - 1 also writes the end
+// entry parts[ceil(C / EXPAND_TILE)] = n - 1.  Writes stay below `cap`.
+__device__ __forceinline__ void markTileStart(u32 *__restrict__ parts, u64 cap, u32 excl, u32 incl, u64 p, u64 n)
+{
+    const u64 t = ((u64)excl + EXPAND_TILE - 1) / EXPAND_TILE;
+    if (t * EXPAND_TILE < incl && t < cap) parts[t] = (u32)p;
+    if (p == n - 1) {
+        const u64 te = ((u64)incl + EXPAND_TILE - 1) / EXPAND_TILE;
+        if (te < cap) parts[te] = (u32)p;
+    }
+}
+
+// scanApplyTile with the grouped offsets: the tile's offset is its group's
+// offset plus its offset inside the group.  With `parts` non-null it also
+// writes the expand partitions (markTileStart).
+__device__ __forceinline__ void scanApplyTileGrouped(u32 *__restrict__ d, u64 n, const u32 *__restrict__ tileIn,
+                                                     const u64 *__restrict__ groupOffs, u64 tile,
+                                                     u32 *__restrict__ parts, u64 partsCap)
+{
+    const int lane = threadIdx.x & 63;
+    if (tile * BS >= n) return;
+    const u32 tileOff = (u32)groupOffs[tile / SCAN_GROUP] + tileIn[tile];
+    const u64 i0 = tile * BS + (u64)lane * 4;
+    const bool vec = i0 + 4 <= n && ((uintptr_t)d & 15) == 0;
+    u32 x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    if (vec) {
+        uint4 q = *(const uint4 *)(d + i0);
+        x0 = q.x, x1 = q.y, x2 = q.z, x3 = q.w;
+    } else {
+        if (i0 < n) x0 = d[i0];
+        if (i0 + 1 < n) x1 = d[i0 + 1];
+        if (i0 + 2 < n) x2 = d[i0 + 2];
+        if (i0 + 3 < n) x3 = d[i0 + 3];
+    }
+    u32 s1 = x0 + x1, s2 = s1 + x2, s3 = s2 + x3;
+    u32 base = tileOff + waveInclScan(s3) - s3;
+    if (vec) {
+        *(uint4 *)(d + i0) = make_uint4(base + x0, base + s1, base + s2, base + s3);
+    } else {
+        if (i0 < n) d[i0] = base + x0;
+        if (i0 + 1 < n) d[i0 + 1] = base + s1;
+        if (i0 + 2 < n) d[i0 + 2] = base + s2;
+        if (i0 + 3 < n) d[i0 + 3] = base + s3;
+    }
+    if (parts) {
+        // entries past n have no children (x = 0) and mark nothing
+        markTileStart(parts, partsCap, base, base + x0, i0, n);
+        markTileStart(parts, partsCap, base + x0, base + s1, i0 + 1, n);
+        markTileStart(parts, partsCap, base + s1, base + s2, i0 + 2, n);
+        markTileStart(parts, partsCap, base + s2, base + s3, i0 + 3, n);
+    }
 }
 
 // One wave turns the 256 counts of `tile` into inclusive counts: lane l holds

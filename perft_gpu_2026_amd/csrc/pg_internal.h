@@ -78,6 +78,10 @@ struct Profiler {
     std::vector<hipEvent_t> pool;
     double ms[K_COUNT] = {}, bytes[K_COUNT] = {}, items[K_COUNT] = {};
     uint64_t launches[K_COUNT] = {};
+    // the launches with the most algorithmic bytes (the largest level of a
+    // repeated call): their bytes, summed time and number
+    double bigBytes[K_COUNT] = {}, bigMs[K_COUNT] = {};
+    uint64_t bigN[K_COUNT] = {};
     int cur = -1;
     hipEvent_t curA = nullptr;
 
@@ -117,6 +121,8 @@ struct Profiler {
             bytes[r.id] += r.bytes;
             items[r.id] += r.items;
             launches[r.id]++;
+            if (r.bytes > bigBytes[r.id]) bigBytes[r.id] = r.bytes, bigMs[r.id] = 0, bigN[r.id] = 0;
+            if (r.bytes == bigBytes[r.id]) bigMs[r.id] += t, bigN[r.id]++;
             pool.push_back(r.a);
             pool.push_back(r.b);
         }
@@ -124,7 +130,8 @@ struct Profiler {
     }
     void reset()
     {
-        for (int i = 0; i < K_COUNT; i++) ms[i] = bytes[i] = items[i] = 0, launches[i] = 0;
+        for (int i = 0; i < K_COUNT; i++)
+            ms[i] = bytes[i] = items[i] = bigBytes[i] = bigMs[i] = 0, launches[i] = bigN[i] = 0;
     }
 };
 

@@ -4,8 +4,11 @@
 // One BFS level (entries = (parent index, move) pairs):
 //   k_makecount   child board = make(parent board, move); count its moves;
 //                 per-256-entry tile sums for the scan            (VALU bound)
-//   k_scan_top    exclusive scan of the tile sums (one block)
-//   k_scan_apply  wave64 block scan + tile offset -> inclusive counts (HBM)
+//   k_scan_groups in-group tile offsets and sums of 64-tile groups
+//   k_scan_top    exclusive scan of the group sums (one block)
+//   k_scan_small  both in one block, for levels of at most 64 groups
+//   k_scan_apply  wave64 tile scan + group offset + in-group offset
+//                 -> inclusive counts                            (HBM)
 //   k_partitions  one upper_bound per 2048-child tile
 //   k_expand      parent index of every child: first-child scatter + max-scan
 //                 in LDS, coalesced non-temporal 16-byte stores    (HBM)
@@ -57,19 +60,50 @@ __global__ __launch_bounds__(BS) void k_tile_sums(const u32 *__restrict__ d, u64
     if (threadIdx.x == 0) tileSums[blockIdx.x] = (u32)s;
 }
 
-// exclusive scan of `tiles` tile sums into tileOffs[0..tiles] (one block)
-__global__ __launch_bounds__(TOP_BS) void k_scan_top(const u32 *__restrict__ tileSums, u64 tiles,
-                                                     u64 *__restrict__ tileOffs)
+// in-group tile offsets and group sums, one wave per four groups
+__global__ __launch_bounds__(BS) void k_scan_groups(const u32 *__restrict__ tileSums, u64 tiles,
+                                                    u32 *__restrict__ tileIn, u32 *__restrict__ groupSums)
 {
-    scanTileSums(tileSums, tiles, tileOffs);
+    scanGroups(tileSums, tiles, tileIn, groupSums, (u64)blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
 }
 
-// Inclusive counts from the tile offsets: one wave per 256-entry tile, four
-// consecutive entries per lane (one 16-byte load and store), a lane-local
-// scan of the four and one wave scan of the lane totals; no block barrier.
-__global__ __launch_bounds__(BS) void k_scan_apply(u32 *__restrict__ d, u64 n, const u64 *__restrict__ tileOffs)
+// Levels of at most 64 groups (4,096 tiles, 1,048,576 entries): the group
+// pass and the scan of the group sums in one block, one launch fewer
+__global__ __launch_bounds__(TOP_BS) void k_scan_small(const u32 *__restrict__ tileSums, u64 tiles,
+                                                       u32 *__restrict__ tileIn, u64 *__restrict__ groupOffs)
 {
-    scanApplyTile(d, n, tileOffs, (u64)blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
+    __shared__ u32 gs[SCAN_GROUP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const u64 G = scanGroupCount(tiles);
+    if (threadIdx.x < SCAN_GROUP) gs[threadIdx.x] = 0;
+    __syncthreads();
+    scanGroups(tileSums, tiles, tileIn, gs, (u64)w);  // 16 waves x 4 groups
+    __syncthreads();
+    if (w == 0) {
+        const u32 v = gs[lane], inc = waveInclScan(v);  // at most 64 groups of 64 * 256 * 218
+        if ((u64)lane < G) groupOffs[lane] = inc - v;
+        if (lane == 63) groupOffs[G] = inc;
+    }
+}
+
+// exclusive scan of the G group sums into groupOffs[0..G] (one block)
+__global__ __launch_bounds__(TOP_BS) void k_scan_top(const u32 *__restrict__ groupSums, u64 groups,
+                                                     u64 *__restrict__ groupOffs)
+{
+    scanTileSums<u64, 2>(groupSums, groups, groupOffs);
+}
+
+// Inclusive counts from the group and in-group offsets: one wave per
+// 256-entry tile, four consecutive entries per lane (one 16-byte load and
+// store), a lane-local scan of the four and one wave scan of the lane totals;
+// no block barrier.  With `parts` non-null it also writes the partitions of
+// the next expand (what k_partitions computes by binary search).
+__global__ __launch_bounds__(BS) void k_scan_apply(u32 *__restrict__ d, u64 n, const u32 *__restrict__ tileIn,
+                                                   const u64 *__restrict__ groupOffs, u32 *__restrict__ parts,
+                                                   u64 partsCap)
+{
+    scanApplyTileGrouped(d, n, tileIn, groupOffs, (u64)blockIdx.x * (BS / 64) + (threadIdx.x >> 6), parts,
+                         partsCap);
 }
 
 // parts[t] = parent of child t*EXPAND_TILE, for t in [0, tiles]
@@ -522,13 +556,41 @@ static int launchMakecount(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx
     return PG_OK;
 }
 
-// scan: tileSums (per 256) -> tileOffs -> in-place inclusive scan
-static int launchScan(u32 *d, u64 n, const u32 *tileSums, u64 *tileOffs, hipStream_t st)
+// scan: tileSums (per 256) -> group offsets -> in-place inclusive scan.
+// offs needs scanTotalIndex(n) + 1 u64 (at most ceil(n/256) + 1); the total
+// lands in offs[scanTotalIndex(n)].
+// Scan buffers of an n-entry level: group offsets (G + 1 u64, the total at
+// [G]), in-group tile offsets (T u32) and group sums (G u32), T = ceil(n/256),
+// G = ceil(T/64).  With `parts` (capacity expandPartsCap(n)) the scan also
+// writes the next expand's partitions, for launchExpandParts.
+struct ScanBufs {
+    u64 *groupOffs;
+    u32 *tileIn, *groupSums;
+};
+static u64 scanTiles(u64 n) { return (n + BS - 1) / BS; }
+static u64 scanTotalIndex(u64 n) { return scanGroupCount(scanTiles(n)); }
+static u64 expandPartsCap(u64 n) { return (n * MAX_MOVES + EXPAND_TILE - 1) / EXPAND_TILE + 1; }
+static bool allocScanBufs(Bump &A, u64 n, ScanBufs &b)
+{
+    const u64 T = scanTiles(n), G = scanGroupCount(T);
+    b.groupOffs = A.alloc<u64>(G + 1);
+    b.tileIn = A.alloc<u32>(T);
+    b.groupSums = A.alloc<u32>(G);
+    return b.groupOffs && b.tileIn && b.groupSums;
+}
+static int launchScan(u32 *d, u64 n, const u32 *tileSums, const ScanBufs &b, hipStream_t st, u32 *parts = nullptr)
 {
     if (!n) return PG_OK;
-    u64 tiles = (n + BS - 1) / BS;
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_BS), 0, st, tileSums, tiles, tileOffs);
-    hipLaunchKernelGGL(k_scan_apply, dim3(gridFor(n, 4 * BS)), dim3(BS), 0, st, d, n, tileOffs);
+    const u64 T = scanTiles(n), G = scanGroupCount(T);
+    if (G <= (u64)SCAN_GROUP) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(TOP_BS), 0, st, tileSums, T, b.tileIn, b.groupOffs);
+    } else {
+        hipLaunchKernelGGL(k_scan_groups, dim3((unsigned)((G + 15) / 16)), dim3(BS), 0, st, tileSums, T, b.tileIn,
+                           b.groupSums);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_BS), 0, st, b.groupSums, G, b.groupOffs);
+    }
+    hipLaunchKernelGGL(k_scan_apply, dim3(gridFor(n, 4 * BS)), dim3(BS), 0, st, d, n, b.tileIn, b.groupOffs, parts,
+                       parts ? expandPartsCap(n) : 0ull);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
@@ -538,6 +600,16 @@ static int launchExpand(const u32 *incl, u64 n, u64 c, u32 *parts, u32 *parentId
     if (!c) return PG_OK;
     u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
     hipLaunchKernelGGL(k_partitions, dim3(gridFor(tiles + 1, BS)), dim3(BS), 0, st, incl, n, c, tiles, parts);
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)tiles), dim3(BS), 0, st, incl, n, c, parts, parentIdx);
+    PG_HIP(hipGetLastError());
+    return PG_OK;
+}
+
+// k_expand on partitions the scan wrote
+static int launchExpandParts(const u32 *incl, u64 n, u64 c, const u32 *parts, u32 *parentIdx, hipStream_t st)
+{
+    if (!c) return PG_OK;
+    u64 tiles = (c + EXPAND_TILE - 1) / EXPAND_TILE;
     hipLaunchKernelGGL(k_expand, dim3((unsigned)tiles), dim3(BS), 0, st, incl, n, c, parts, parentIdx);
     PG_HIP(hipGetLastError());
     return PG_OK;
@@ -646,12 +718,13 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         u64 *known = A.alloc<u64>(n);
         u64 *val = A.alloc<u64>(n);
         u32 *tileSums = A.alloc<u32>(tiles);
-        u64 *tileOffs = A.alloc<u64>(tiles + 1);
+        ScanBufs sb;
+        const bool sbOk = allocScanBufs(A, n, sb);
         u32 *redir = A.alloc<u32>(n);
         u64 tslots = 1;
         while (tslots < 2 * n) tslots <<= 1;
         DedupEntry *tab = A.alloc<DedupEntry>(tslots);
-        if (!ob || !os || !cnt || !known || !val || !tileSums || !tileOffs || !redir || !tab) return PG_ERR_ARENA_OOM;
+        if (!ob || !os || !cnt || !known || !val || !tileSums || !sbOk || !redir || !tab) return PG_ERR_ARENA_OOM;
         dim3 g(gridFor(n, BS)), gc(tallyGrid(n)), b(BS);
         int rd = depth - lvl;
         PG_HIP(hipMemsetAsync(tab, 0, tslots * sizeof(DedupEntry), st));
@@ -667,8 +740,8 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
                                tileSums);
         }
         PG_HIP(hipGetLastError());
-        PG_TRY(launchScan(cnt, n, tileSums, tileOffs, st));
-        PG_HIP(hipMemcpyAsync(pinned, tileOffs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
+        PG_TRY(launchScan(cnt, n, tileSums, sb, st));
+        PG_HIP(hipMemcpyAsync(pinned, sb.groupOffs + scanTotalIndex(n), sizeof(u64), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         u64 c = pinned[0];
         lv.push_back({ob, os, cnt, known, redir, val, n});
@@ -793,16 +866,18 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
         u8 *os = A.alloc<u8>(n);
         u32 *cnt = A.alloc<u32>(n);
         u32 *tileSums = A.alloc<u32>(tiles);
-        u64 *tileOffs = A.alloc<u64>(tiles + 1);
-        if (!ob || !os || !cnt || !tileSums || !tileOffs) return PG_ERR_ARENA_OOM;
+        ScanBufs sb;
+        const bool sbOk = allocScanBufs(A, n, sb);
+        u32 *parts = A.alloc<u32>(expandPartsCap(n));
+        if (!ob || !os || !cnt || !tileSums || !sbOk || !parts) return PG_ERR_ARENA_OOM;
         // algorithmic bytes: 43N + 33P (make+count), 8N (scan)
         P.begin(K_MAKECOUNT, st);
         PG_TRY(launchMakecount(pb, ps, idx, mv, pc, n, ob, os, cnt, tileSums, st));
         P.end(43.0 * n + 33.0 * nParents, (double)n, st);
         P.begin(K_SCAN, st);
-        PG_TRY(launchScan(cnt, n, tileSums, tileOffs, st));
+        PG_TRY(launchScan(cnt, n, tileSums, sb, st, parts));
         P.end(8.0 * n, (double)n, st);
-        PG_HIP(hipMemcpyAsync(pinned, tileOffs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipMemcpyAsync(pinned, sb.groupOffs + scanTotalIndex(n), sizeof(u64), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         u64 c = pinned[0];
         if (c == 0) {
@@ -812,13 +887,12 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
             return PG_OK;
         }
         if (c >= 0xFFFFFFFFull) return PG_ERR_ARENA_OOM;
-        u32 *parts = A.alloc<u32>((c + EXPAND_TILE - 1) / EXPAND_TILE + 1);
         u32 *nidx = A.alloc<u32>(c);
         u16 *nmv = A.alloc<u16>(c);
-        if (!parts || !nidx || !nmv) return PG_ERR_ARENA_OOM;
+        if (!nidx || !nmv) return PG_ERR_ARENA_OOM;
         // 4N + 4C (expand), 37N + 2C (genmoves)
         P.begin(K_EXPAND, st);
-        PG_TRY(launchExpand(cnt, n, c, parts, nidx, st));
+        PG_TRY(launchExpandParts(cnt, n, c, parts, nidx, st));
         P.end(4.0 * n + 4.0 * c, (double)c, st);
         P.begin(K_GENMOVES, st);
         PG_TRY(launchGenmoves(ob, os, cnt, pc ^ 1, n, nmv, st));
@@ -877,18 +951,22 @@ extern "C" int pg_scan_u32(uint32_t *data, uint64_t n, const uint32_t *tileSums,
         if (total) *total = 0;
         return PG_OK;
     }
-    u64 tiles = (n + BS - 1) / BS;
-    u64 *offs = (u64 *)scratch;
+    // scratch: group offsets (G + 1 u64), in-group tile offsets (T u32),
+    // group sums (G u32), then the tile sums when the caller has none (T u32)
+    const u64 T = scanTiles(n), G = scanGroupCount(T);
+    ScanBufs sb;
+    sb.groupOffs = (u64 *)scratch;
+    sb.tileIn = (u32 *)(sb.groupOffs + G + 1);
+    sb.groupSums = sb.tileIn + T;
     const u32 *ts = tileSums;
     if (!ts) {
-        // tile sums go after the offsets in scratch (caller sized it for both)
-        u32 *own = (u32 *)(offs + tiles + 1);
+        u32 *own = sb.groupSums + G;
         hipLaunchKernelGGL(k_tile_sums, dim3(gridFor(n, BS)), dim3(BS), 0, st, data, n, own);
         ts = own;
     }
-    PG_TRY(launchScan(data, n, ts, offs, st));
+    PG_TRY(launchScan(data, n, ts, sb, st));
     if (total) {
-        PG_HIP(hipMemcpyAsync(lib().hostPinned, offs + tiles, sizeof(u64), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipMemcpyAsync(lib().hostPinned, sb.groupOffs + G, sizeof(u64), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         *total = lib().hostPinned[0];
     }
@@ -956,6 +1034,9 @@ extern "C" int pg_profile_get(pg_kernel_stat *out, int maxOut, int *nOut)
         s.ms = P.ms[k];
         s.bytes = P.bytes[k];
         s.items = P.items[k];
+        s.big_bytes = P.bigBytes[k];
+        s.big_ms = P.bigN[k] ? P.bigMs[k] / P.bigN[k] : 0;
+        s.big_launches = P.bigN[k];
     }
     *nOut = n;
     return PG_OK;

@@ -144,13 +144,13 @@ def _d(gpu, a):
     return gpu.DeviceArray.from_numpy(a)
 
 
-@pytest.mark.parametrize("n", [1, 255, 256, 257, 100_000, 3_000_001, 4_194_305, 9_000_000])
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 16_385, 100_000, 3_000_001, 4_194_305, 9_000_000, 67_110_147])
 def test_scan_matches_cumsum(gpu, n):
     rng = np.random.default_rng(n)
     host = rng.integers(0, 219, size=n, dtype=np.uint32)
     d = _d(gpu, host)
     tiles = (n + 255) // 256
-    scratch = gpu.DeviceArray(8 * (tiles + 1) + 4 * tiles + 256)
+    scratch = gpu.DeviceArray(8 * (tiles + 4) + 4 * tiles)  # the documented size
     total = ctypes.c_uint64()
     assert gpu.lib().pg_scan_u32(d.ptr, n, None, scratch.ptr, ctypes.byref(total), None) == 0
     gpu.device_synchronize()
@@ -158,10 +158,12 @@ def test_scan_matches_cumsum(gpu, n):
     assert total.value == int(host.sum(dtype=np.uint64))
 
 
-@pytest.mark.parametrize("n,shift", [(9_000_000, 0), (9_000_000, 1), (700_000, 1)])
+@pytest.mark.parametrize("n,shift", [(9_000_000, 0), (9_000_000, 1), (700_000, 1), (67_110_147, 1)])
 def test_scan_with_given_tile_sums(gpu, n, shift):
     # caller-provided tile sums, 16-byte aligned (shift 0) or not (shift 1: 4 bytes
-    # in), over several passes of the tile-offset scan (16,384 tile sums each)
+    # in); groups of 64 tile sums, and the top scan takes rows of 4,096 group
+    # sums (67,108,864 entries), so 67,110,147 entries take two rows, the
+    # second partial
     rng = np.random.default_rng(n + shift)
     host = rng.integers(0, 219, size=n, dtype=np.uint32)
     tiles = (n + 255) // 256
@@ -169,7 +171,7 @@ def test_scan_with_given_tile_sums(gpu, n, shift):
     sums = np.concatenate([host, pad]).reshape(tiles, 256).sum(axis=1, dtype=np.uint64).astype(np.uint32)
     d = _d(gpu, host)
     ts = _d(gpu, np.concatenate([np.zeros(shift, dtype=np.uint32), sums]))
-    scratch = gpu.DeviceArray(8 * (tiles + 1) + 256)
+    scratch = gpu.DeviceArray(8 * (tiles + 4))  # the documented size
     total = ctypes.c_uint64()
     ptr = ctypes.c_void_p(ts.ptr.value + 4 * shift)
     assert gpu.lib().pg_scan_u32(d.ptr, n, ptr, scratch.ptr, ctypes.byref(total), None) == 0
