@@ -416,7 +416,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
                    "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
-        "roofline": roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None) if leaf else None,
+        "roofline": dict(roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None),
+                         note="the leaf is integer VALU work with no contraction: its bound is VALU issue "
+                              "(valu_issue), so its HBM fraction is small by nature") if leaf else None,
         "roofline_levels": roofline(top_level) if top_level else None,
         "valu_issue": valu_issue(leaf) if leaf and a.leaf2_variant == 0 else None,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),

@@ -721,10 +721,12 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         ScanBufs sb;
         const bool sbOk = allocScanBufs(A, n, sb);
         u32 *redir = A.alloc<u32>(n);
+        u32 *parts = A.alloc<u32>(expandPartsCap(n));
         u64 tslots = 1;
         while (tslots < 2 * n) tslots <<= 1;
         DedupEntry *tab = A.alloc<DedupEntry>(tslots);
-        if (!ob || !os || !cnt || !known || !val || !tileSums || !sbOk || !redir || !tab) return PG_ERR_ARENA_OOM;
+        if (!ob || !os || !cnt || !known || !val || !tileSums || !sbOk || !redir || !parts || !tab)
+            return PG_ERR_ARENA_OOM;
         dim3 g(gridFor(n, BS)), gc(tallyGrid(n)), b(BS);
         int rd = depth - lvl;
         PG_HIP(hipMemsetAsync(tab, 0, tslots * sizeof(DedupEntry), st));
@@ -740,7 +742,7 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
                                tileSums);
         }
         PG_HIP(hipGetLastError());
-        PG_TRY(launchScan(cnt, n, tileSums, sb, st));
+        PG_TRY(launchScan(cnt, n, tileSums, sb, st, parts));
         PG_HIP(hipMemcpyAsync(pinned, sb.groupOffs + scanTotalIndex(n), sizeof(u64), hipMemcpyDeviceToHost, st));
         PG_HIP(hipStreamSynchronize(st));
         u64 c = pinned[0];
@@ -751,12 +753,11 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
             break;
         }
         if (c >= 0xFFFFFFFFull) return PG_ERR_ARENA_OOM;
-        u32 *parts = A.alloc<u32>((c + EXPAND_TILE - 1) / EXPAND_TILE + 1);
         u32 *nidx = A.alloc<u32>(c);
         u16 *nmv = A.alloc<u16>(c);
-        if (!parts || !nidx || !nmv) return PG_ERR_ARENA_OOM;
+        if (!nidx || !nmv) return PG_ERR_ARENA_OOM;
         if (c) {
-            PG_TRY(launchExpand(cnt, n, c, parts, nidx, st));
+            PG_TRY(launchExpandParts(cnt, n, c, parts, nidx, st));
             PG_TRY(launchGenmoves(ob, os, cnt, pc ^ 1, n, nmv, st));
         }
         pb = ob;
