@@ -78,7 +78,7 @@ __device__ __forceinline__ u64 blockSum64(u64 v)
 // = total, by one block of TOP_BS threads.  The sums are taken in rows of
 // 4 * TOP_BS: thread t reads sums 4t..4t+3 of a row with one 16-byte load, so
 // each wave load covers 1 KB of consecutive sums; a wave scan of the thread
-// totals and the waves' totals in LDS give every sum its offset.  TOP_ROWS
+// totals and the waves' totals in LDS give every sum its offset.  ROWS
 // rows share one barrier and have their loads in flight together.
 // ACC is the type the waves' totals of a row are added in: u32 for tile sums
 // (a row of 4,096 at most 256 * 218 each), u64 for group sums (64 tiles each).
@@ -87,18 +87,17 @@ constexpr int TOP_ROWS = 8;
 template <typename ACC = u32, int ROWS = TOP_ROWS>
 __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u64 tiles, u64 *__restrict__ tileOffs)
 {
-    constexpr int TOP_ROWS = ROWS;
-    __shared__ u32 wtot[TOP_ROWS][TOP_BS / 64];
+    __shared__ u32 wtot[ROWS][TOP_BS / 64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    constexpr u64 ROW = 4ull * TOP_BS, PASS = ROW * TOP_ROWS;
+    constexpr u64 ROW = 4ull * TOP_BS, PASS = ROW * ROWS;
     const bool aligned = ((uintptr_t)tileSums & 15) == 0;
     u64 carry = 0;
     for (u64 c0 = 0; c0 < tiles; c0 += PASS) {
         const u64 left = tiles - c0;
-        const int rows = left >= PASS ? TOP_ROWS : (int)((left + ROW - 1) / ROW);  // uniform
-        u32 x[TOP_ROWS][4], ex[TOP_ROWS];
+        const int rows = left >= PASS ? ROWS : (int)((left + ROW - 1) / ROW);  // uniform
+        u32 x[ROWS][4], ex[ROWS];
 #pragma unroll
-        for (int r = 0; r < TOP_ROWS; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const u64 i0 = c0 + r * ROW + 4u * tid;
             x[r][0] = x[r][1] = x[r][2] = x[r][3] = 0;
             if (r < rows) {
@@ -115,7 +114,7 @@ __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u
         // a thread's four sums and a wave's 256 fit u32 for tile sums (at
         // most 256 * 218) and group sums (at most 64 tiles of those)
 #pragma unroll
-        for (int r = 0; r < TOP_ROWS; r++) {
+        for (int r = 0; r < ROWS; r++) {
             if (r < rows) {
                 u32 t = x[r][0] + x[r][1] + x[r][2] + x[r][3];
                 u32 incl = waveInclScan(t);
@@ -126,7 +125,7 @@ __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u
         __syncthreads();
         u64 rowBase = carry;
 #pragma unroll
-        for (int r = 0; r < TOP_ROWS; r++) {
+        for (int r = 0; r < ROWS; r++) {
             if (r < rows) {
                 ACC before = 0, all = 0;
 #pragma unroll
