@@ -150,7 +150,8 @@ int pg_tt_free(void);           /* freeTT */
 int pg_tt_stats(uint64_t out[6]);
 /* TT snapshot / resume (no reference counterpart; its 27.8 h perft 14 run had
  * no resume, README.md:63): write both tiers to a file, or load a file into
- * tables created by pg_tt_init with the same sizes */
+ * tables created by pg_tt_init with the same sizes (a file that fails part
+ * way through leaves the tables cleared) */
 int pg_tt_save(const char *path);
 int pg_tt_load(const char *path);
 

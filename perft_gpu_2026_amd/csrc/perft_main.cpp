@@ -5,7 +5,10 @@
 // the reference's form: "Perft(%02d): <count>, time: <s> seconds, nps: <n>".
 // Extra flags of this engine: -leaf <1|2> (leaf plies of the GPU BFS),
 // -arena <MB> (BFS arena size), -divide (per-root-move counts at <depth>),
-// -suite <file> (EPD perft suite: "fen ;D1 n1 ;D2 n2 ..." per line).
+// -suite <file> (EPD perft suite: "fen ;D1 n1 ;D2 n2 ..." per line),
+// -ttsave <file> (write both table tiers to <file> after every finished depth)
+// and -ttload <file> (fill the tables from such a file before the first
+// depth; the tables must have the sizes the file was written with).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -127,18 +130,35 @@ static int runSuite(const char *path, int maxDepth, bool cpu, bool tt)
     return failures ? 1 : 0;
 }
 
+// Table snapshots for long runs (no reference counterpart: its perft 14 ran
+// 27.8 h with no resume, README.md:63).  A failed load leaves the tables
+// empty, so the run continues from scratch.
+static void loadSnapshot(const char *path)
+{
+    if (!path) return;
+    if (pg_tt_load(path) == PG_OK) printf("TT snapshot loaded: %s\n", path);
+    else printf("warning: TT snapshot not loaded: %s\n", pg_last_error());
+    fflush(stdout);
+}
+static void saveSnapshot(const char *path)
+{
+    if (path && pg_tt_save(path) != PG_OK) printf("warning: TT snapshot not saved: %s\n", pg_last_error());
+}
+
 int main(int argc, char *argv[])
 {
     bool cpuMode = false, useTT = true, divide = false;
-    const char *suite = nullptr;
+    const char *suite = nullptr, *ttSave = nullptr, *ttLoad = nullptr;
     long ldOverride = 0, dttMB = 0, httMB = 0, arenaMB = 0, leaf = 0;
     // flags anywhere (perft.cu:17-57): consume them and compact argv
     for (int i = 1; i < argc; i++) {
         long *valueFlag = nullptr;
         if (!strcmp(argv[i], "-cpu")) cpuMode = true;
         else if (!strcmp(argv[i], "-divide")) divide = true;
-        else if (!strcmp(argv[i], "-suite") && i + 1 < argc) {
-            suite = argv[i + 1];
+        else if ((!strcmp(argv[i], "-suite") || !strcmp(argv[i], "-ttsave") || !strcmp(argv[i], "-ttload")) &&
+                 i + 1 < argc) {
+            const char *&dst = argv[i][3] == 'i' ? suite : argv[i][3] == 's' ? ttSave : ttLoad;
+            dst = argv[i + 1];
             for (int j = i; j + 2 < argc; j++) argv[j] = argv[j + 2];
             argc -= 2;
             i--;
@@ -217,12 +237,16 @@ int main(int argc, char *argv[])
     if (cpuMode) {
         printf("CPU mode\n");
         fflush(stdout);
-        if (useTT) pg_tt_init(0, httMB ? (uint64_t)httMB : 1024);
+        if (useTT) {
+            pg_tt_init(0, httMB ? (uint64_t)httMB : 1024);
+            loadSnapshot(ttLoad);
+        }
         for (int d = 1; d <= maxDepth; d++) {
             uint64_t c = 0;
             double t0 = now();
             pg_perft_cpu(&pos, d, useTT, &c);
             printResult(d, c, 0, now() - t0);
+            if (useTT) saveSnapshot(ttSave);
         }
         pg_tt_free();
         return 0;
@@ -247,6 +271,7 @@ int main(int argc, char *argv[])
         // 288 GB device (the library halves it where it does not fit)
         uint64_t dMB = dttMB ? (uint64_t)dttMB : 65536, hMB = httMB ? (uint64_t)httMB : 4096;
         if (pg_tt_init(dMB, hMB) != PG_OK) printf("warning: TT init failed: %s\n", pg_last_error());
+        loadSnapshot(ttLoad);
     }
     int rc = 0;
     for (int d = 1; d <= maxDepth; d++) {
@@ -262,6 +287,7 @@ int main(int argc, char *argv[])
             break;
         }
         printResult(d, r.lo, r.hi, t1 - t0);
+        if (useTT) saveSnapshot(ttSave);
         uint64_t ooms = pg_get_oom_fallback_count() - oomBefore;
         if (ooms > 0 && d >= ld) {
             int newLD = pg_get_effective_ld();

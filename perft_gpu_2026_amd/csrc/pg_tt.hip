@@ -265,7 +265,9 @@ int ttLoad(const char *path)
     if (ok && buckets) g_host.used = used ? used : 1;
     fclose(f);
     if (!ok) {
-        setError("reading %s failed", path);
+        // a part-filled table would answer with wrong counts: start empty
+        ttClear();
+        setError("reading %s failed (tables cleared)", path);
         return PG_ERR_INVALID;
     }
     return PG_OK;

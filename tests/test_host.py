@@ -180,6 +180,27 @@ def test_cli_suite_cpu(engine, tmp_path):
     assert r.returncode == 1 and "1 failures" in r.stdout
 
 
+def test_cli_tt_snapshot_flags_cpu(engine, tmp_path):
+    # -ttsave writes the tables after every depth; -ttload resumes from them.
+    # Counts are the published Kiwipete ones either way; a truncated snapshot
+    # is refused with a warning and the run goes on from empty tables.
+    fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    want = {1: 48, 2: 2039, 3: 97862, 4: 4085603}
+    snap = tmp_path / "tt.bin"
+    r = engine.run_cli([fen, "4", "-cpu", "-htt", "16", "-ttsave", str(snap)])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert engine.parse_cli_counts(r.stdout) == want
+    assert snap.stat().st_size > 0
+    r = engine.run_cli([fen, "4", "-cpu", "-htt", "16", "-ttload", str(snap)])
+    assert r.returncode == 0 and "TT snapshot loaded" in r.stdout, r.stdout
+    assert engine.parse_cli_counts(r.stdout) == want
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(snap.read_bytes()[: snap.stat().st_size // 2])
+    r = engine.run_cli([fen, "4", "-cpu", "-htt", "16", "-ttload", str(cut)])
+    assert r.returncode == 0 and "warning: TT snapshot not loaded" in r.stdout, r.stdout
+    assert engine.parse_cli_counts(r.stdout) == want
+
+
 def test_host_tt_snapshot_resume(engine, tmp_path):
     # host tier only (no GPU here): save after a run, reload into fresh tables,
     # and the reloaded table answers the same queries exactly
