@@ -264,6 +264,21 @@ def test_cli_gpu(gpu):
     assert gpu.parse_cli_counts(r.stdout)[6] == 119060324
 
 
+def test_cli_gpu_tt_snapshot(gpu, tmp_path):
+    # the CLI writes both tiers after every depth and a second run resumes
+    # from the file: same counts, and the run says it loaded the snapshot
+    start = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
+    snap = tmp_path / "tt.bin"
+    args = [start, "6", "-arena", "4096", "-dtt", "64", "-htt", "64"]
+    r = gpu.run_cli(args + ["-ttsave", str(snap)])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert gpu.parse_cli_counts(r.stdout)[6] == 119060324
+    assert snap.stat().st_size > 32 << 20
+    r = gpu.run_cli(args + ["-ttload", str(snap)])
+    assert r.returncode == 0 and "TT snapshot loaded" in r.stdout, r.stdout + r.stderr
+    assert gpu.parse_cli_counts(r.stdout) == {1: 20, 2: 400, 3: 8902, 4: 197281, 5: 4865609, 6: 119060324}
+
+
 def test_genmoves_writes_nothing_for_zero_count_entries(gpu):
     # entries whose count is 0 (a table hit or a duplicate in TT mode) still
     # have legal moves; genmoves must not write them over their neighbours'
