@@ -433,17 +433,21 @@ __global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
 // of leaf entries and entries are root-ordered) sums its pair counts in
 // registers and adds one wave total to the root; a chunk that straddles roots
 // keeps per-entry partial sums in LDS (atomics) and attributes them per lane.
-// The choice is wave-uniform, so the pair loop does not diverge.
-constexpr int BL2_CAP = 2048;
+// The choice is wave-uniform, so the pair loop does not diverge.  Pairs take
+// 3 bytes of LDS and the kernels 128 VGPRs at most, so four blocks fit a CU
+// and four waves a SIMD, as for k_leaf2q.
+constexpr int BL2_CAP = 2016;
 constexpr int BL2_WAVES = 4;
+#define BL2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 4)))
 template <int C>
-__global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
+__global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2(BatchDev *d)
 {
     if (d->oom) return;
     __shared__ QuadBitBoard sb[BL2_WAVES][B_CHUNK];
     __shared__ u8 ss[BL2_WAVES][B_CHUNK];
     __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
-    __shared__ u32 sl[BL2_WAVES][BL2_CAP];
+    __shared__ u16 slm[BL2_WAVES][BL2_CAP];
+    __shared__ u8 sll[BL2_WAVES][BL2_CAP];
     const u32 L = d->leafLevel;
     // descriptor fields into registers once (the atomics below would otherwise
     // force them to be re-read from the descriptor)
@@ -485,21 +489,21 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2(BatchDev *d)
             if (cnt && off < wb + BL2_CAP && off + cnt > wb) {
                 QuadBitBoard q = sb[w][lane];
                 u8 st = ss[w][lane];
-                u32 *dst = sl[w];
-                const u32 tag = (u32)lane << 16;
+                u16 *dm = slm[w];
+                u8 *dl = sll[w];
                 generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;
-                    if (j < BL2_CAP) dst[j] = (u32)m | tag;
+                    if (j < BL2_CAP) dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();
             u32 wn = total - wb < (u32)BL2_CAP ? total - wb : (u32)BL2_CAP;
             for (u32 j = lane; j < wn; j += 64) {
-                u32 x = sl[w][j];
-                u32 src = x >> 16;
+                u16 x = slm[w][j];
+                u32 src = sll[w][j];
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
-                makeMove<1 - C>(c, s2, (u16)x);
+                makeMove<1 - C>(c, s2, x);
                 u32 cm = countMoves<C>(c, s2);
                 if (oneRoot) acc += cm;
                 else atomicAdd(&sacc[w][src], cm);  // LDS atomic
@@ -544,13 +548,14 @@ __global__ __launch_bounds__(BS) void k_b_leaf1v(BatchDev *d)
 // the other's stored count (a single-root perft 7 made 25 % more leaf misses
 // in order).
 template <int C>
-__global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
+__global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(BatchDev *d)
 {
     if (d->oom) return;
     __shared__ QuadBitBoard sb[BL2_WAVES][B_CHUNK];
     __shared__ u8 ss[BL2_WAVES][B_CHUNK];
     __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
-    __shared__ u32 sl[BL2_WAVES][BL2_CAP];
+    __shared__ u16 slm[BL2_WAVES][BL2_CAP];
+    __shared__ u8 sll[BL2_WAVES][BL2_CAP];
     const u32 L = d->leafLevel;
     const LvlDev a = d->lv[L];
     const LvlDev p = d->lv[L - 1];
@@ -588,21 +593,21 @@ __global__ __launch_bounds__(BL2_WAVES * 64) void k_b_leaf2v(BatchDev *d)
             if (cnt && off < wb + BL2_CAP && off + cnt > wb) {
                 QuadBitBoard q = sb[w][lane];
                 u8 st = ss[w][lane];
-                u32 *dst = sl[w];
-                const u32 tag = (u32)lane << 16;
+                u16 *dm = slm[w];
+                u8 *dl = sll[w];
                 generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;
-                    if (j < BL2_CAP) dst[j] = (u32)m | tag;
+                    if (j < BL2_CAP) dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();
             u32 wn = total - wb < (u32)BL2_CAP ? total - wb : (u32)BL2_CAP;
             for (u32 j = lane; j < wn; j += 64) {
-                u32 x = sl[w][j];
-                u32 src = x >> 16;
+                u16 x = slm[w][j];
+                u32 src = sll[w][j];
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
-                makeMove<1 - C>(c, s2, (u16)x);
+                makeMove<1 - C>(c, s2, x);
                 atomicAdd(&sacc[w][src], countMoves<C>(c, s2));
             }
             waveLdsSync();

@@ -201,9 +201,14 @@ __global__ __launch_bounds__(BS) void k_leaf1(const QuadBitBoard *__restrict__ p
 // Pair lists longer than the LDS capacity are processed in windows.  A chunk
 // of 64 entries keeps every lane busy in the child phase (make, count,
 // generate); 64 entries have about 1,750 pairs, so 2,048 pair slots rarely
-// need a second window.  41 KB of LDS per 4-wave block, 3 blocks per CU.
+// need a second window.  A pair takes 3 bytes of LDS (the move, and the lane
+// of its child in a separate byte array): 32 KB per 4-wave block, so four
+// blocks fit a CU, and the kernels are held to 128 VGPRs, so four waves fit a
+// SIMD (140 VGPRs, 4-byte pairs and three waves per SIMD measured 3.6 %
+// slower; five waves need 96 VGPRs, which spill, and measured 11 % slower).
+#define L2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 4)))
 constexpr int L2_CHUNK = 64;
-constexpr int L2_CAP = 2048;
+constexpr int L2_CAP = 2016;
 constexpr int L2_WAVES = 4;
 
 // Chunks are taken either by a static stride over the grid, or (QUEUE) from
@@ -217,7 +222,8 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
     static_assert(CHUNK <= 64, "one entry per lane");
     __shared__ QuadBitBoard sb[L2_WAVES][CHUNK];
     __shared__ u8 ss[L2_WAVES][CHUNK];
-    __shared__ u32 sl[L2_WAVES][CAP];
+    __shared__ u16 slm[L2_WAVES][CAP];
+    __shared__ u8 sll[L2_WAVES][CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     u64 acc = 0;
     const u64 stride = (u64)gridDim.x * L2_WAVES * CHUNK;
@@ -252,21 +258,21 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
             if (cnt && off < wb + CAP && off + cnt > wb) {
                 QuadBitBoard q = sb[w][lane];
                 u8 st = ss[w][lane];
-                u32 *dst = sl[w];
-                const u32 tag = (u32)lane << 16;
+                u16 *dm = slm[w];
+                u8 *dl = sll[w];
                 generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;  // wraps (and is skipped) below the window
-                    if (j < (u32)CAP) dst[j] = (u32)m | tag;
+                    if (j < (u32)CAP) dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();
             u32 wn = total - wb < (u32)CAP ? total - wb : (u32)CAP;
             for (u32 j = lane; j < wn; j += 64) {
-                u32 x = sl[w][j];
-                u32 src = x >> 16;
+                u16 x = slm[w][j];
+                u32 src = sll[w][j];
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
-                makeMove<1 - C>(c, s2, (u16)x);
+                makeMove<1 - C>(c, s2, x);
                 acc += countMoves<C>(c, s2);
             }
             waveLdsSync();
@@ -280,14 +286,14 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
 // with a 1,536-pair list (half the child phase's lanes idle; measured 8 %
 // slower on the config-2 tree).  pg_set_leaf2_variant picks.
 template <int C>
-__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2(const QuadBitBoard *__restrict__ pb,
+__global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2(const QuadBitBoard *__restrict__ pb,
                                                          const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
                                                          const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum)
 {
     leaf2Body<C, L2_CHUNK, L2_CAP, false>(pb, ps, pidx, mv, n, sum, nullptr);
 }
 template <int C>
-__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2q(const QuadBitBoard *__restrict__ pb,
+__global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2q(const QuadBitBoard *__restrict__ pb,
                                                           const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
                                                           const u16 *__restrict__ mv, u64 n, u64 *__restrict__ sum,
                                                           u64 *work)
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2q(const QuadBitBoard *__
     leaf2Body<C, L2_CHUNK, L2_CAP, true>(pb, ps, pidx, mv, n, sum, work);
 }
 template <int C>
-__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_c32(const QuadBitBoard *__restrict__ pb,
+__global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_c32(const QuadBitBoard *__restrict__ pb,
                                                              const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
                                                              const u16 *__restrict__ mv, u64 n,
                                                              u64 *__restrict__ sum)
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(BS) void k_leaf1_val(const QuadBitBoard *__restrict
 // plain stride, and the permutation took startpos perft 10 with the tables
 // from 0.91 to 0.84 s.
 template <int C>
-__global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard *__restrict__ pb,
+__global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const QuadBitBoard *__restrict__ pb,
                                                              const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
                                                              const u16 *__restrict__ mv, u64 n, u64 *__restrict__ val,
                                                              TTEntry *__restrict__ tt, u64 mask, u64 *ctr)
@@ -414,7 +420,8 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard 
     __shared__ QuadBitBoard sb[L2_WAVES][L2_CHUNK];
     __shared__ u8 ss[L2_WAVES][L2_CHUNK];
     __shared__ u32 sacc[L2_WAVES][L2_CHUNK];
-    __shared__ u32 sl[L2_WAVES][L2_CAP];
+    __shared__ u16 slm[L2_WAVES][L2_CAP];
+    __shared__ u8 sll[L2_WAVES][L2_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     TTTally tally;
     // chunk k of each wave's sequence is chunk k * P mod nChunks of the level
@@ -446,21 +453,21 @@ __global__ __launch_bounds__(L2_WAVES * 64) void k_leaf2_val(const QuadBitBoard 
             if (cnt && off < wb + L2_CAP && off + cnt > wb) {
                 QuadBitBoard q = sb[w][lane];
                 u8 st = ss[w][lane];
-                u32 *dst = sl[w];
-                const u32 tag = (u32)lane << 16;
+                u16 *dm = slm[w];
+                u8 *dl = sll[w];
                 generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;
-                    if (j < L2_CAP) dst[j] = (u32)m | tag;
+                    if (j < L2_CAP) dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();
             u32 wn = total - wb < (u32)L2_CAP ? total - wb : (u32)L2_CAP;
             for (u32 j = lane; j < wn; j += 64) {
-                u32 x = sl[w][j];
-                u32 src = x >> 16;
+                u16 x = slm[w][j];
+                u32 src = sll[w][j];
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
-                makeMove<1 - C>(c, s2, (u16)x);
+                makeMove<1 - C>(c, s2, x);
                 atomicAdd(&sacc[w][src], countMoves<C>(c, s2));
             }
             waveLdsSync();
