@@ -79,27 +79,27 @@ def cpu_baseline(fen: str, depth: int, budget_s: float = 15.0) -> dict:
 # kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0);
 # the writes are mostly the chunk queue's atomics.
 # Commands and figures: profiles/leaf2_config2_pmc.md.
-PMC_TRAFFIC = {"leaf2": 66224.0 * 1024 * 2.0 + 7586.3 * 1024 * 1.0}
+PMC_TRAFFIC = {"leaf2": 66226.3 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
 
 # VALU work per launch of the same leaf: SQ_INSTS_VALU (wave64 instructions)
 # and SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64), the share of lanes
 # active per VALU instruction, from a third rocprofv3 --pmc run of this bench
 # (profiles/leaf2_config2_pmc.md).  Divided by the live launch time, the
 # instruction count gives the issue rate per SIMD and cycle (1,024 SIMDs at
-# the 2.4 GHz peak clock); one wave64 VALU instruction per 4 cycles per SIMD
-# is 0.25.
-PMC_VALU = {"leaf2": (4.6467e9, 0.91)}
+# the 2.4 GHz peak clock).
+PMC_VALU = {"leaf2": (4.6990e9, 0.906)}
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
-def valu_issue(stat: dict):
+def valu_issue(stat: dict, counts: int):
+    """counts: the leaf's make + countMoves calls per launch (perft(depth - 1))."""
     if stat["name"] not in PMC_VALU:
         return None
     insts, lanes = PMC_VALU[stat["name"]]
     avg_s = stat["ms"] / stat["launches"] * 1e-3
     return {"kernel": stat["name"], "wave_insts_per_launch": insts, "lanes_active": lanes,
             "wave_insts_per_simd_cycle": round(insts / (SIMDS * CLOCK_HZ * avg_s), 4),
-            "one_per_4_cycles": 0.25}
+            "lane_insts_per_count": round(insts * 64 / counts)}
 
 
 def roofline(stat: dict, traffic=None) -> dict:
@@ -417,10 +417,10 @@ def main():
         "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
                    "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
         "roofline": dict(roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None),
-                         note="the leaf is integer VALU work with no contraction: its bound is VALU issue "
-                              "(valu_issue), so its HBM fraction is small by nature") if leaf else None,
+                         note="the leaf is integer VALU work with no contraction: it is bound by its VALU "
+                              "instruction stream (valu_issue), so its HBM fraction is small by nature") if leaf else None,
         "roofline_levels": roofline(top_level) if top_level else None,
-        "valu_issue": valu_issue(leaf) if leaf and a.leaf2_variant == 0 else None,
+        "valu_issue": valu_issue(leaf, fx["counts"][depth - 2]) if leaf and a.leaf2_variant == 0 else None,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
                     for k, v in stats.items()},
