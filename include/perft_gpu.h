@@ -131,7 +131,7 @@ void pg_set_leaf_plies(int plies);
 /* shape of the fused two-ply leaf: 0 = 64 entries per wave (default),
  * 1 = 32 entries per wave */
 void pg_set_leaf2_variant(int variant);
-/* grid of the grid-stride leaf kernels in blocks per CU (default 8; the one-ply
+/* grid of the grid-stride leaf kernels in blocks per CU (default 16; the one-ply
  * leaf uses twice as many) -- a tuning knob of this engine, no reference
  * counterpart */
 void pg_set_leaf_grid(int blocksPerCU);

@@ -309,7 +309,10 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_c32(const 
     leaf2Body<C, 32, 1536, false>(pb, ps, pidx, mv, n, sum, nullptr);
 }
 int g_leaf2Variant = 0;
-int g_leafBlocksPerCU = 8;  // grid of the grid-stride leaves, per CU (x2 for the one-ply leaf)
+// grid of the grid-stride leaves, per CU (x2 for the one-ply leaf): with four
+// resident two-ply blocks per CU, 16 (four rounds) took the tables' startpos
+// perft 11 from 9.78 to 9.52 s and config 4 from 2.21 to 2.16 s against 8
+int g_leafBlocksPerCU = 16;
 
 // ------------------------------------------------------------ TT-mode kernels
 // With a transposition table each expanded level probes the table at its
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(BS) void k_leaf1_val(const QuadBitBoard *__restrict
 // two-ply leaf with per-entry values and the depth-2 table (the reference's
 // leaf TT, perft_kernels.cu:867-877): wave-cooperative like k_leaf2, with each
 // entry's partial sum kept in LDS.  Each wave visits chunks by a static stride
-// over a grid of up to 8 blocks per CU, through a fixed permutation of the
+// over a grid of up to 16 blocks per CU, through a fixed permutation of the
 // level's chunks, so the chunks in flight together are spread over the level.
 // In-order chunks put the copies of a position that nearby parents share
 // (A-then-B and B-then-A) in flight together, where none of them finds the
@@ -1084,4 +1087,4 @@ extern "C" int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind)
 }
 
 extern "C" void pg_set_leaf2_variant(int v) { g_leaf2Variant = v == 1 ? 1 : 0; }
-extern "C" void pg_set_leaf_grid(int blocksPerCU) { g_leafBlocksPerCU = blocksPerCU >= 1 && blocksPerCU <= 256 ? blocksPerCU : 8; }
+extern "C" void pg_set_leaf_grid(int blocksPerCU) { g_leafBlocksPerCU = blocksPerCU >= 1 && blocksPerCU <= 256 ? blocksPerCU : 16; }
