@@ -1,7 +1,8 @@
 """Benchmark: perft nodes/s on one MI355X.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--leaf 1|2]
-                  [--no-batch] [--no-tt] [--serve N] [--serve-rate R]
+                  [--no-levels] [--no-batch] [--no-tt] [--no-ref-positions]
+                  [--serve N] [--serve-rate R]
 
 Workload (one step): perft(7) of the config-2 position -- the start position
 after 8 seeded random plies (seed 2, SplitMix64 walk over canonically sorted
@@ -22,15 +23,17 @@ The JSON line carries:
                 `roofline_levels_full_size` gives scan and expand on the
                 largest level of the tree (config 2 run with the one-ply leaf).
   valu_issue    the leaf's VALU issue rate per SIMD and cycle: its measured
-                instruction count per launch over the live launch time.
+                instruction count per launch over the live launch time, and
+                its lane instructions per counted position.
   cpu_baseline  the oracle (a C restatement of the reference's single-threaded
                 -cpu path) on this host, one core, on a bounded sample of the
                 same tree: perft(6) under the first root moves.
-After the timed config-2 steps (the `value`) it runs three more legs, each
-checked against golden counts: the 4096-position config-3 batch, batched vs
-plain; config 4 (64 positions, perft 8) with the transposition tables,
-reporting table probes/s; the reference README's published positions
-(startpos perft 9, Kiwipete perft 7, Position 3 perft 10, no TT) with the
+After the timed config-2 steps (the `value`) it runs more legs, each checked
+against golden counts: the level kernels at full size (above); the
+4096-position config-3 batch, batched vs plain; config 4 (64 positions,
+perft 8) with the transposition tables, reporting table probes/s; the
+reference README's published positions (startpos perft 9, Kiwipete perft 7,
+Position 3 perft 10 without the tables; startpos perft 9-11 with them) with the
 published times beside them; and the config-5 serving loop, replaying N requests
 (default all 10,000) with Poisson arrivals (rate R requests/s; default 0.75 x
 the capacity measured by a closed-loop pass), reporting p50/p99 latency and
