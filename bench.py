@@ -18,7 +18,7 @@ The JSON line carries:
                 and the algorithmic bytes 6K + 33P per launch (K leaf entries,
                 P parent boards).  The leaf is VALU-bound integer work, so
                 its HBM fraction is small by nature; `roofline_levels` gives
-                the same for the BFS level kernels, which are HBM-bound, at
+                the same for the HBM-bound BFS level kernels (scan, expand) at
                 config 2's own level sizes (small: launch-latency bound), and
                 `roofline_levels_full_size` gives scan and expand on the
                 largest level of the tree (config 2 run with the one-ply leaf).
@@ -402,7 +402,8 @@ def main():
     dt = t1 - t0
     value = want * a.steps / dt
     leaf = stats.get("leaf2") or stats.get("leaf1")
-    levels = [v for k, v in stats.items() if k in ("makecount", "scan", "expand", "genmoves")]
+    # the HBM-bound level kernels (make+count and genmoves are VALU-bound)
+    levels = [v for k, v in stats.items() if k in ("scan", "expand")]
     top_level = max(levels, key=lambda v: v["ms"]) if levels else None
     out = {
         "metric": "perft nodes/s on 1 MI355X, batched vs plain BFS; p50/p99 query latency",
