@@ -264,6 +264,26 @@ def test_cli_gpu(gpu):
     assert gpu.parse_cli_counts(r.stdout)[6] == 119060324
 
 
+def test_cli_gpu_divide_and_suite(gpu, tmp_path):
+    # -divide on the GPU: one line per root move, and the total is perft(depth)
+    kiwi = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    r = gpu.run_cli([kiwi, "5", "-divide", "-arena", "4096"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l for l in r.stdout.splitlines() if len(l) > 4 and l[4] in ":qrbn"]
+    assert len(lines) == 48
+    assert sum(int(l.split(":")[1]) for l in lines) == 193690690
+    assert "Total: 193690690" in r.stdout
+    # an EPD suite on the GPU, one line failing on purpose
+    f = tmp_path / "suite.epd"
+    f.write_text(kiwi + " ;D1 48 ;D2 2039 ;D3 97862 ;D4 4085603\n"
+                 "8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - ;D1 14 ;D2 191 ;D3 2812 ;D4 43238 ;D5 674624\n")
+    r = gpu.run_cli(["-suite", str(f), "5", "-arena", "4096"])
+    assert r.returncode == 0 and "9 checks, 0 failures" in r.stdout, r.stdout + r.stderr
+    f.write_text(kiwi + " ;D1 48 ;D2 2040\n")
+    r = gpu.run_cli(["-suite", str(f), "5", "-arena", "4096"])
+    assert r.returncode == 1 and "1 failures" in r.stdout, r.stdout
+
+
 def test_cli_gpu_tt_snapshot(gpu, tmp_path):
     # the CLI writes both tiers after every depth and a second run resumes
     # from the file: same counts, and the run says it loaded the snapshot
