@@ -7,9 +7,10 @@
  *
  * The reference exposes its engine as C++ functions in launcher.h:1-47 plus the
  * `perft_gpu` command line (perft.cu:13-186).  Each entry point below names the
- * reference interface it replaces.  The C++ host API with the reference's own
- * names (initGPU, perftLauncher, perft_gpu_host_bfs, ...) lives in
- * perft_gpu_2026_amd/csrc/pg_launcher.h and is built on these entry points.
+ * reference interface it replaces.  The C++ interface with the reference's own
+ * names and signatures (initGPU, perftLauncher, perft_gpu_host_bfs, the
+ * g_lastBfsOom global, ...) is declared in include/perft_launcher.h and is
+ * compiled into the same library on top of these entry points.
  *
  * Position encoding (chess.h:66-92): bb[0] = black occupancy, bb[1..3] = bits
  * 0..2 of the piece code (P=1 N=2 B=3 R=4 Q=5 K=6); state = white castle in
@@ -235,9 +236,13 @@ int pg_perft_batch_plain(int n, const pg_position *positions, const uint8_t *dep
 typedef struct pg_serve_config {
     uint64_t hbmBudgetBytes;  /* BFS buffers + TT */
     uint64_t ttBytes;         /* part of the budget given to the shared device TT */
-    int maxInflight;          /* requests on the device at once */
-    int maxBatch;             /* requests merged into one device batch */
-    int reserved[4];
+    int maxInflight;          /* requests on the device at once (slots, at most 16) */
+    int maxBatch;             /* queued requests of one depth a free slot takes at
+                                 once and runs as one multi-root device pass (1 =
+                                 one request per pass) */
+    int maxQueued;            /* requests waiting beyond which pg_serve_submit
+                                 returns PG_ERR_BUSY (0 = no limit) */
+    int reserved[3];
 } pg_serve_config;
 typedef struct pg_serve_result {
     uint64_t id;
@@ -246,7 +251,13 @@ typedef struct pg_serve_result {
     int      status;
     int      _pad;
 } pg_serve_result;
+/* One server at a time: while it exists its slots own the library arena and
+ * share the device table, so pg_perft_bfs, pg_perft, pg_perft_batch,
+ * pg_perft_batch_plain, pg_shutdown, the pg_tt_* management calls and a
+ * second pg_serve_create return PG_ERR_BUSY until pg_serve_destroy. */
 int pg_serve_create(const pg_serve_config *cfg, void **handle);
+/* PG_ERR_BUSY when maxQueued requests already wait, PG_ERR_INVALID for a bad
+ * position or a depth outside 0..16 */
 int pg_serve_submit(void *handle, uint64_t id, const pg_position *pos, int depth);
 /* copies up to maxResults finished results; waits up to timeoutMs for one */
 int pg_serve_poll(void *handle, pg_serve_result *results, int maxResults, int timeoutMs, int *nOut);

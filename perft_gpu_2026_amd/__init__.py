@@ -55,7 +55,7 @@ class BatchOptions(ctypes.Structure):
 
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbmBudgetBytes", c_uint64), ("ttBytes", c_uint64), ("maxInflight", c_int), ("maxBatch", c_int),
-                ("reserved", c_int * 4)]
+                ("maxQueued", c_int), ("reserved", c_int * 3)]
 
 
 class ServeResult(ctypes.Structure):

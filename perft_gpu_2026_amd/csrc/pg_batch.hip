@@ -921,6 +921,7 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
         return PG_ERR_NOT_INIT;
     }
     std::lock_guard<std::mutex> lock(L.mu);
+    PG_TRY(refuseWhileServing("pg_perft_batch"));
     if (!g_batch.dev) PG_HIP(hipMalloc((void **)&g_batch.dev, sizeof(BatchDev)));
     bool useGraph = !opt || opt->useGraph;
     bool useTT = opt && opt->useTT && L.tt.entries;
@@ -981,6 +982,7 @@ int pg_perft_batch(int n, const pg_position *positions, const uint8_t *depths, c
                    uint64_t *counts)
 {
     if (n < 0 || (n && (!positions || !depths || !counts))) return PG_ERR_INVALID;
+    for (int i = 0; i < n; i++) PG_TRY(validatePosition(&positions[i]));
     if (n == 0) return PG_OK;
     return perftBatch(n, positions, depths, opt, counts);
 }

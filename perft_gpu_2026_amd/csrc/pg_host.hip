@@ -34,10 +34,36 @@ Lib &lib()
     return L;
 }
 
+int refuseWhileServing(const char *what)
+{
+    if (!lib().serving.load()) return PG_OK;
+    setError("%s: a server owns the arena and the device table until pg_serve_destroy", what);
+    return PG_ERR_BUSY;
+}
+
+int validatePosition(const pg_position *p)
+{
+    if (!p) {
+        setError("position is NULL");
+        return PG_ERR_INVALID;
+    }
+    const u64 b1 = p->bb[1], b2 = p->bb[2], b3 = p->bb[3], occ = b1 | b2 | b3;
+    const u64 kings = b2 & b3 & ~b1;
+    const char *why = nullptr;
+    if (p->color > 1) why = "side to move is not 0 or 1";
+    else if (b1 & b2 & b3) why = "piece code 7 on a square";
+    else if (p->bb[0] & ~occ) why = "black pieces outside the occupied squares";
+    else if (popc(kings & p->bb[0]) != 1 || popc(kings & ~p->bb[0]) != 1) why = "not exactly one king per side";
+    if (!why) return PG_OK;
+    setError("invalid position: %s", why);
+    return PG_ERR_INVALID;
+}
+
 // launcher state (launcher.cu:565-568)
 static int g_effectiveLD = 0;
 static std::atomic<uint64_t> g_oomCount{0};
 static size_t g_lastPeak = 0;
+static size_t g_peakSinceReset = 0;  // largest g_lastPeak since resetPeakSinceReset
 static uint64_t g_arenaLimit = 0;  // 0 = whole arena; a test hook to force OOM
 int g_leafPlies = 2;  // fused two-ply leaf (measured 2x the one-ply leaf on config 2)
 static bool g_frontierBatch = true;  // launcher: batch the frontier above the launch depth
@@ -104,11 +130,7 @@ static int parseFen(const char *fen, pg_position *pos)
     while (fen[i] == ' ') i++;
     if (fen[i] >= 'a' && fen[i] <= 'h') pos->state |= (u8)((fen[i] - 'a' + 1) << 4);
     // exactly one king per side is required by the move generator
-    QuadBitBoard q = toQBB(pos);
-    u64 kings = q.bb[2] & q.bb[3] & ~q.bb[1];
-    u64 black = q.bb[0];
-    if (popc(kings & black) != 1 || popc(kings & ~black) != 1) return PG_ERR_INVALID;
-    return PG_OK;
+    return validatePosition(pos);
 }
 
 static int writeFen(const pg_position *pos, char *out, int outLen)
@@ -281,12 +303,16 @@ int gpuBfs(const pg_position *pos, int depth, bool useTT, void *buf, uint64_t by
         return PG_ERR_NOT_INIT;
     }
     std::lock_guard<std::mutex> g(L.mu);
+    // checked under the lock: pg_serve_create sets the flag, then takes the
+    // lock, so a call that holds the lock first finishes before any slot runs
+    PG_TRY(refuseWhileServing("GPU perft"));
     size_t peak = 0;
     size_t avail = L.arenaBytes;
     if (g_arenaLimit && g_arenaLimit < avail) avail = g_arenaLimit;
     int s = bfsPerft(toQBB(pos), pos->state, pos->color, depth, g_leafPlies, useTT, buf ? buf : L.arena,
                      buf ? bytes : avail, L.stream, L.hostPinned, count, &peak);
     g_lastPeak = peak;
+    if (peak > g_peakSinceReset) g_peakSinceReset = peak;
     return s;
 }
 
@@ -526,6 +552,8 @@ int getEffectiveLD() { return g_effectiveLD; }
 void setEffectiveLD(int ld) { g_effectiveLD = ld; }
 uint64_t getOomFallbackCount() { return g_oomCount.load(); }
 size_t lastBfsPeak() { return g_lastPeak; }
+size_t peakSinceReset() { return g_peakSinceReset; }
+void resetPeakSinceReset() { g_peakSinceReset = 0; }
 size_t usableArena()
 {
     size_t a = lib().arenaBytes;
@@ -542,7 +570,11 @@ extern "C" {
 const char *pg_last_error(void) { return g_err.c_str(); }
 
 int pg_init(int device, uint64_t arenaBytes) { return initLib(device, arenaBytes); }
-int pg_shutdown(void) { return shutdownLib(); }
+int pg_shutdown(void)
+{
+    PG_TRY(refuseWhileServing("pg_shutdown"));
+    return shutdownLib();
+}
 uint64_t pg_arena_bytes(void) { return lib().arenaBytes; }
 void pg_set_arena_limit(uint64_t bytes) { g_arenaLimit = bytes; }
 void pg_set_frontier_batch(int on) { g_frontierBatch = on != 0; }
@@ -550,27 +582,30 @@ void pg_set_frontier_batch(int on) { g_frontierBatch = on != 0; }
 int pg_parse_fen(const char *fen, pg_position *pos) { return parseFen(fen, pos); }
 int pg_write_fen(const pg_position *pos, char *out, int outLen)
 {
-    if (!pos || !out) return PG_ERR_INVALID;
+    if (!out) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     return writeFen(pos, out, outLen);
 }
 
 int pg_gen_moves(const pg_position *pos, uint16_t *moves, int *n)
 {
-    if (!pos || !moves || !n) return PG_ERR_INVALID;
+    if (!moves || !n) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     *n = (int)generateMoves(toQBB(pos), pos->state, pos->color, moves);
     return PG_OK;
 }
 
 int pg_count_moves(const pg_position *pos, int *n)
 {
-    if (!pos || !n) return PG_ERR_INVALID;
+    if (!n) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     *n = (int)countMoves(toQBB(pos), pos->state, pos->color);
     return PG_OK;
 }
 
 int pg_make_move(pg_position *pos, uint16_t move)
 {
-    if (!pos) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     QuadBitBoard q = toQBB(pos);
     makeMove(q, pos->state, pos->color, move);
     memcpy(pos->bb, q.bb, 32);
@@ -599,14 +634,17 @@ int pg_random_request(uint64_t seed, int pliesLo, int pliesHi, int depthLo, int 
 
 int pg_perft_bfs(const pg_position *pos, int depth, int useTT, void *gpuBuffer, uint64_t bufferBytes, uint64_t *count)
 {
-    if (!pos || !count || depth < 0 || depth > 20) return PG_ERR_INVALID;
+    if (!count || depth < 0 || depth > 20) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     return gpuBfs(pos, depth, useTT != 0, gpuBuffer, bufferBytes, count);
 }
 
 int pg_perft(const pg_position *pos, int depth, int launchDepth, int useTT, pg_u128 *count)
 {
-    if (!pos || !count) return PG_ERR_INVALID;
+    if (!count) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     if (!lib().init) return PG_ERR_NOT_INIT;
+    PG_TRY(refuseWhileServing("pg_perft"));
     u128 r;
     int s = perftLaunch(pos, depth, launchDepth, useTT != 0, &r);
     count->lo = r.lo;
@@ -616,13 +654,15 @@ int pg_perft(const pg_position *pos, int depth, int launchDepth, int useTT, pg_u
 
 int pg_perft_cpu(const pg_position *pos, int depth, int useTT, uint64_t *count)
 {
-    if (!pos || !count) return PG_ERR_INVALID;
+    if (!count) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     return perftCpu(pos, depth, useTT != 0, count);
 }
 
 int pg_estimate_launch_depth(const pg_position *pos, float *bf, int *ld)
 {
-    if (!pos || !bf || !ld) return PG_ERR_INVALID;
+    if (!bf || !ld) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     return estimateLD(pos, bf, ld);
 }
 
@@ -634,6 +674,14 @@ uint64_t pg_last_bfs_peak_bytes(void) { return lastBfsPeak(); }
 int pg_perft_batch_plain(int n, const pg_position *positions, const uint8_t *depths, int useTT, uint64_t *counts)
 {
     if (n < 0 || (n && (!positions || !depths || !counts))) return PG_ERR_INVALID;
+    for (int i = 0; i < n; i++) {
+        PG_TRY(validatePosition(&positions[i]));
+        if (depths[i] > 20) {
+            setError("pg_perft_batch_plain: depth %d out of range", depths[i]);
+            return PG_ERR_INVALID;
+        }
+    }
+    if (n) PG_TRY(refuseWhileServing("pg_perft_batch_plain"));
     for (int i = 0; i < n; i++) {
         u128 r;
         // one GPU BFS call per position; a position too big for the arena
@@ -650,6 +698,5 @@ int pg_perft_batch_plain(int n, const pg_position *positions, const uint8_t *dep
     }
     return PG_OK;
 }
-
 
 }  // extern "C"

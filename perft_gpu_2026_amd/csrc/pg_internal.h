@@ -146,8 +146,17 @@ struct Lib {
     DeviceTT tt;
     int cuCount = 0;
     std::mutex mu;  // serialises use of the arena / stream by host threads
+    // set while a server exists: its slots own the arena and share the device
+    // table, so calls that use either return PG_ERR_BUSY until it is destroyed
+    std::atomic<bool> serving{false};
 };
 Lib &lib();
+// PG_ERR_BUSY (with the error text naming `what`) while a server exists, else PG_OK
+int refuseWhileServing(const char *what);
+// PG_OK if the position is one the move generator can take: exactly one king
+// per side, black pieces a subset of the occupied squares, no piece code 7,
+// side to move 0 or 1; else PG_ERR_INVALID with the reason as the error text
+int validatePosition(const pg_position *p);
 
 // kernels (pg_kernels.hip)
 // Blocks of kernel `fn` (block size `blockSize`) resident on the whole device

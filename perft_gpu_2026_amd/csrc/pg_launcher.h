@@ -1,6 +1,6 @@
-// pg_launcher.h -- C++ host API with the reference's launcher names
-// (launcher.h:1-47), for C++ callers of the reference.  Implemented on the
-// same code as the C ABI in include/perft_gpu.h.
+// pg_launcher.h -- the library's internal host API (launcher, GPU BFS, -cpu
+// path, batch) that the C ABI in include/perft_gpu.h and the reference-named
+// C++ layer in include/perft_launcher.h are built on.
 #pragma once
 #include <stdint.h>
 
@@ -36,6 +36,9 @@ int getEffectiveLD();
 void setEffectiveLD(int ld);
 uint64_t getOomFallbackCount();
 size_t lastBfsPeak();
+// largest per-call arena peak since resetPeakSinceReset (printTTStats)
+size_t peakSinceReset();
+void resetPeakSinceReset();
 // arena bytes a GPU call may use (the arena, or the test limit if set)
 size_t usableArena();
 extern int g_leafPlies;

@@ -113,6 +113,7 @@ struct Server {
     }
 };
 
+static int serveCreateLocked(const pg_serve_config *cfg, void **handle);
 static int serveCreate(const pg_serve_config *cfg, void **handle)
 {
     Lib &L = lib();
@@ -120,6 +121,23 @@ static int serveCreate(const pg_serve_config *cfg, void **handle)
         setError("pg_init has not run");
         return PG_ERR_NOT_INIT;
     }
+    // one server at a time; from here until pg_serve_destroy the slots own
+    // the arena and the table, and every other call that uses them is refused
+    bool expected = false;
+    if (!L.serving.compare_exchange_strong(expected, true)) {
+        setError("a server already exists");
+        return PG_ERR_BUSY;
+    }
+    // wait for a GPU call of another thread that started before the flag was set
+    std::lock_guard<std::mutex> drain(L.mu);
+    int s = serveCreateLocked(cfg, handle);
+    if (s != PG_OK) L.serving = false;
+    return s;
+}
+
+static int serveCreateLocked(const pg_serve_config *cfg, void **handle)
+{
+    Lib &L = lib();
     Server *S = new Server();
     S->cfg = *cfg;
     L.prof.on = false;  // the event profiler is single-threaded; slots run concurrently
@@ -185,6 +203,7 @@ static int serveDestroy(Server *S)
         L.tt = DeviceTT{};
     }
     delete S;
+    lib().serving = false;
     return PG_OK;
 }
 
@@ -202,11 +221,19 @@ int pg_serve_create(const pg_serve_config *cfg, void **handle)
 
 int pg_serve_submit(void *handle, uint64_t id, const pg_position *pos, int depth)
 {
-    if (!handle || !pos || depth < 0 || depth > 16) return PG_ERR_INVALID;
+    if (!handle || depth < 0 || depth > 16) return PG_ERR_INVALID;
+    PG_TRY(validatePosition(pos));
     Server *S = (Server *)handle;
     {
         std::lock_guard<std::mutex> lk(S->mu);
-        if (S->stop) return PG_ERR_BUSY;
+        if (S->stop) {
+            setError("the server is shutting down");
+            return PG_ERR_BUSY;
+        }
+        if (S->cfg.maxQueued > 0 && S->queue.size() >= (size_t)S->cfg.maxQueued) {
+            setError("%d requests already wait (maxQueued)", S->cfg.maxQueued);
+            return PG_ERR_BUSY;
+        }
         S->queue.push_back({id, *pos, depth, nowSec()});
     }
     S->cvWork.notify_one();

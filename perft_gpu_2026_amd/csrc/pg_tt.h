@@ -123,6 +123,7 @@ void hostTTStore(const Hash128 &h, int depth, uint64_t count);
 int ttInit(uint64_t deviceMB, uint64_t hostMB);
 int ttFree();
 int ttClear();
+int ttClearDevice();  // the device table only (clearDeviceTTs)
 int ttStats(uint64_t out[6]);
 void ttCountDevice(u64 probes, u64 hits, u64 stores);
 

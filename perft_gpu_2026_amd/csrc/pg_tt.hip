@@ -153,13 +153,19 @@ int ttFree()
     return PG_OK;
 }
 
-int ttClear()
+int ttClearDevice()
 {
     Lib &L = lib();
     if (L.tt.entries) {
         PG_HIP(hipMemsetAsync(L.tt.entries, 0, (L.tt.mask + 1) * 16, L.stream));
         PG_HIP(hipStreamSynchronize(L.stream));
     }
+    return PG_OK;
+}
+
+int ttClear()
+{
+    PG_TRY(ttClearDevice());
     if (g_host.heads) {
         memset((void *)g_host.heads, 0, (g_host.mask + 1) * sizeof(u32));
         g_host.used = 1;
@@ -262,14 +268,24 @@ int ttLoad(const char *path)
     }
     if (ok && buckets) ok = fread((void *)g_host.heads, sizeof(u32), buckets, f) == buckets;
     if (ok && used) ok = fread(g_host.pool, sizeof(HostEntry), used, f) == used;
-    if (ok && buckets) g_host.used = used ? used : 1;
     fclose(f);
-    if (!ok) {
+    // every chain link must stay inside the entries in use (index 0 ends a
+    // chain): the probe follows them, so a corrupt or edited file could send
+    // it outside the pool
+    bool linksOk = true;
+    if (ok && buckets) {
+        const u64 lim = used ? used : 1;
+        for (u64 b = 0; b < buckets && linksOk; b++) linksOk = g_host.heads[b].load(std::memory_order_relaxed) < lim;
+        for (u64 i = 1; i < used && linksOk; i++) linksOk = g_host.pool[i].next < lim;
+    }
+    if (!ok || !linksOk) {
         // a part-filled table would answer with wrong counts: start empty
         ttClear();
-        setError("reading %s failed (tables cleared)", path);
+        setError(ok ? "%s holds chain links outside its entries (tables cleared)" : "reading %s failed (tables cleared)",
+                 path);
         return PG_ERR_INVALID;
     }
+    if (buckets) g_host.used = used ? used : 1;
     return PG_OK;
 }
 
@@ -303,9 +319,23 @@ void u128::toDecimal(char *buf, int n) const
 using namespace pg;
 
 extern "C" {
-int pg_tt_init(uint64_t deviceMB, uint64_t hostMB) { return ttInit(deviceMB, hostMB); }
-int pg_tt_clear(void) { return ttClear(); }
-int pg_tt_free(void) { return ttFree(); }
+// the tables are shared with a server's slots while one exists: creating,
+// clearing, freeing, saving or loading them then returns PG_ERR_BUSY
+int pg_tt_init(uint64_t deviceMB, uint64_t hostMB)
+{
+    PG_TRY(refuseWhileServing("pg_tt_init"));
+    return ttInit(deviceMB, hostMB);
+}
+int pg_tt_clear(void)
+{
+    PG_TRY(refuseWhileServing("pg_tt_clear"));
+    return ttClear();
+}
+int pg_tt_free(void)
+{
+    PG_TRY(refuseWhileServing("pg_tt_free"));
+    return ttFree();
+}
 int pg_tt_stats(uint64_t out[6])
 {
     if (!out) return PG_ERR_INVALID;
@@ -314,11 +344,13 @@ int pg_tt_stats(uint64_t out[6])
 int pg_tt_save(const char *path)
 {
     if (!path) return PG_ERR_INVALID;
+    PG_TRY(refuseWhileServing("pg_tt_save"));
     return ttSave(path);
 }
 int pg_tt_load(const char *path)
 {
     if (!path) return PG_ERR_INVALID;
+    PG_TRY(refuseWhileServing("pg_tt_load"));
     return ttLoad(path);
 }
 int pg_u128_to_string(uint64_t lo, uint64_t hi, char *buf, int n)

@@ -22,6 +22,10 @@ Configs (sizes and seeds as the benchmark defines them):
   ep  hand-set positions where a side in check by a double-pushed pawn has two
       en-passant captures (which the reference's perft(1) counts once), and
       positions one ply before them, perft 1..6
+  hm  high-mobility positions: black to move, and after almost every black
+      move white has 81-149 legal moves, so the fused two-ply leaf's children
+      exceed the reference's 80-move leaf cap (switches.h:24,
+      perft_kernels.cu:385) at plies 1 and 3; perft 1..5
 """
 from __future__ import annotations
 
@@ -46,6 +50,13 @@ EP_EVASION = [
     ("full board, before d7-d5+", "rnbqkbnr/ppp1pppp/8/2P1P3/4K3/8/PP1P1PPP/RNBQ1BNR b kq - 0 1"),
 ]
 
+# found by a seeded search over queen placements (black king h8 behind pawns,
+# rook a8, knight b8; white king e1, rooks a1/h1, 4-8 queens)
+HIGH_MOBILITY = [
+    ("eight queens, a", "rn5k/5ppp/1QQQQ3/8/6QQ/3Q4/6Q1/R3K2R b KQ - 0 1"),
+    ("eight queens, b", "rn5k/5ppp/1Q2Q3/3Q4/5Q2/7Q/Q1QQ4/R3K2R b KQ - 0 1"),
+]
+
 
 def dump(name, rows):
     path = os.path.join(HERE, name)
@@ -61,7 +72,7 @@ def main():
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     T = a.threads
-    only = set(a.only.split(",")) if a.only else {"c1", "c2", "c3", "c4", "c5", "ep"}
+    only = set(a.only.split(",")) if a.only else {"c1", "c2", "c3", "c4", "c5", "ep", "hm"}
     t0 = time.time()
 
     if "c1" in only:
@@ -107,6 +118,19 @@ def main():
         with open(path, "w") as f:
             json.dump({"_source": "oracle/oracle.c perft, whose perft(1) counts a checked side's en-passant "
                                   "captures once (MoveGeneratorBitboard.h:1639-1649)", "positions": rows}, f, indent=1)
+        print(f"wrote {path}", flush=True)
+    if "hm" in only:
+        rows = []
+        for name, fen in HIGH_MOBILITY:
+            p = oracle.parse_fen(fen)
+            kids = [oracle.count_moves(oracle.make_move(p, m)) for m in oracle.gen_moves(p)]
+            rows.append({"name": name, "fen": fen, "child_moves_max": max(kids),
+                         "children_above_80": sum(1 for k in kids if k > 80),
+                         "counts": [oracle.perft(p, d, T) for d in range(1, 6)]})
+        path = os.path.join(HERE, "high_mobility.json")
+        with open(path, "w") as f:
+            json.dump({"_source": "oracle/oracle.c perft; positions whose leaf children have more than 80 legal "
+                                  "moves", "positions": rows}, f, indent=1)
         print(f"wrote {path}", flush=True)
     print(f"done in {time.time() - t0:.0f}s", flush=True)
 

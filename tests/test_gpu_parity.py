@@ -83,6 +83,56 @@ def test_ep_evasion_level_kernels(gpu):
         assert int(s.to_numpy(np.uint64, 1)[0]) == want[plies]
 
 
+def test_position3_perft10_reference_count(gpu):
+    # The reference README's Position 3 perft 10 (README.md:74) is the one
+    # published count that includes its once-only en-passant evasion count
+    # (MoveGeneratorBitboard.h:1639-1649): 221,239 fewer leaves than the legal
+    # moves.  A shared mistake of oracle and engine (both evasions counted)
+    # would give 860,322,602,309 instead.
+    k = {p["name"]: p for p in load_json("known_positions.json")["positions"]}["position3"]
+    p = gpu.parse_fen(k["fen"])
+    print("[gpu] position3 perft 10 (launcher, no tables)", flush=True)
+    assert gpu.perft(p, 10) == 860_322_381_070
+    print("[gpu] position3 perft 9 (one BFS call)", flush=True)
+    assert gpu.perft_bfs(p, 9) == k["counts"][8]
+
+
+@pytest.mark.parametrize("leaf", [1, 2])
+def test_leaf_children_above_80_moves(gpu, leaf):
+    # the reference's fused leaf keeps its child's moves in CMove[80]
+    # (switches.h:24, perft_kernels.cu:385); these children have up to 149
+    # moves, so a 64-entry chunk's pair list spans several LDS windows
+    gpu.set_leaf_plies(leaf)
+    try:
+        for h in load_json("high_mobility.json")["positions"]:
+            assert h["child_moves_max"] > 80
+            pos = gpu.parse_fen(h["fen"])
+            print(f"[gpu] high mobility leaf={leaf} {h['name']}", flush=True)
+            assert [gpu.perft_bfs(pos, d) for d in range(1, 6)] == h["counts"], h["name"]
+            assert gpu.perft(pos, 5, launch_depth=3) == h["counts"][4], h["name"]
+    finally:
+        gpu.set_leaf_plies(2)
+    if leaf == 2:
+        # the 32-entry-chunk shape and the leaf entry point on the same children
+        gpu.lib().pg_set_leaf2_variant(1)
+        try:
+            for h in load_json("high_mobility.json")["positions"]:
+                assert gpu.perft_bfs(gpu.parse_fen(h["fen"]), 5) == h["counts"][4]
+        finally:
+            gpu.lib().pg_set_leaf2_variant(0)
+        h = load_json("high_mobility.json")["positions"][0]
+        p = gpu.parse_fen(h["fen"])
+        mv = gpu.gen_moves(p)
+        pb = _d(gpu, np.array([list(p.bb)], dtype=np.uint64))
+        ps = _d(gpu, np.array([p.state], dtype=np.uint8))
+        di = _d(gpu, np.zeros(len(mv), dtype=np.uint32))
+        dm = _d(gpu, np.array(mv, dtype=np.uint16))
+        s = gpu.DeviceArray(8).zero()
+        assert gpu.lib().pg_leaf(pb.ptr, ps.ptr, di.ptr, dm.ptr, 1, len(mv), 2, s.ptr, None) == 0
+        gpu.device_synchronize()
+        assert int(s.to_numpy(np.uint64, 1)[0]) == h["counts"][2]
+
+
 def test_config1_all_depths(gpu):
     r = load_jsonl("c1.jsonl")[0]
     p = gpu.parse_fen(r["fen"])
@@ -321,3 +371,38 @@ def test_genmoves_writes_nothing_for_zero_count_entries(gpu):
         assert sorted(int(x) for x in got[off:off + counts[k]]) == (sorted(gpu.gen_moves(p)) if counts[k] else [])
         off += counts[k]
     assert (got[c:] == 0xFFFF).all()
+
+
+@pytest.mark.parametrize("tables", [True, False])
+def test_reference_launcher_interface(gpu, tables):
+    # a C++ program written only against the reference's launcher names
+    # (include/perft_launcher.h), linked with the library: the whole depth in
+    # one GPU call, a forced OOM reported through g_lastBfsOom, the -cpu
+    # dispatch, and perft.cu's per-depth loop with launch depth 3 (host
+    # recursion above it, with the host tier when the tables are on)
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "bin", "launcher_caller")
+    kiwi = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    want = [48, 2039, 97862, 4085603, 193690690]
+    args = [exe, kiwi, "5", "3"] + ([] if tables else ["-nott"])
+    print(f"[gpu] launcher_caller tables={tables}", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    kv = {}
+    for line in r.stdout.splitlines():
+        parts = line.split()
+        if parts and parts[0] in ("bfs", "bfs_small", "cpu3", "estimate", "oom_fallbacks"):
+            kv.setdefault(parts[0], []).append(parts[1:])
+    assert kv["bfs"][0][:3] == ["193690690", "oom", "0"]
+    assert kv["bfs_small"][0] == ["0", "oom", "1"]
+    assert kv["cpu3"][0] == ["97862"]
+    assert int(kv["estimate"][0][0]) >= 1
+    assert all(v[0] == "0" for v in kv["oom_fallbacks"])
+    counts = gpu.parse_cli_counts(r.stdout)  # the last Perft line of each depth
+    perft_lines = [line for line in r.stdout.splitlines() if line.startswith("Perft(")]
+    # five GPU depths, depth 5 again after clearDeviceTTs, then three -cpu depths
+    assert [int(line.split()[1].rstrip(",")) for line in perft_lines] == want + [want[4]] + want[:3]
+    assert counts[5] == want[4]
+    assert ("Device TT probes" in r.stdout) == tables
+    assert r.stdout.rstrip().endswith("done")

@@ -49,6 +49,51 @@ def test_tt_launcher_with_host_tier(tt):
     assert st[3] > 0 and st[4] > 0  # host tier probed and hit
 
 
+def test_config4_with_64gb_table(gpu):
+    # config 4's 64 positions (10 random plies, seeds 100..163) through the
+    # launcher with a 64 GB device table and a host tier: the oracle's depth-5
+    # counts, then TT on == TT off at depth 6 with two host plies above the
+    # launch depth (host tier probed) and with one whole-depth BFS call each
+    rows = load_jsonl("c4.jsonl")
+    assert len(rows) == 64
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    gpu.tt_init(65536, 2048)
+    try:
+        for r, p in zip(rows, ps):
+            print(f"[gpu] config4 seed {r['seed']} depth {r['golden_depth']} with the tables", flush=True)
+            assert gpu.perft(p, r["golden_depth"], use_tt=True) == r["golden_count"], r["seed"]
+        print("[gpu] config4 depth 6, tables on (launch depth 4) and off", flush=True)
+        on = [gpu.perft(p, 6, launch_depth=4, use_tt=True) for p in ps]
+        on_bfs = [gpu.perft_bfs(p, 6, use_tt=True) for p in ps]
+        off = [gpu.perft_bfs(p, 6) for p in ps]
+        assert on == off and on_bfs == off
+        st = gpu.tt_stats()
+        assert st[1] > 0 and st[3] > 0 and st[4] > 0, st  # device hits, host probes and hits
+    finally:
+        gpu.tt_free()
+
+
+def test_high_mobility_leaf_children_with_tables_and_batch(gpu):
+    # leaf children with 81-149 moves through the TT leaf and the batch leaves
+    hm = load_json("high_mobility.json")["positions"]
+    ps = [gpu.parse_fen(h["fen"]) for h in hm for _ in range(5)]
+    ds = [d for _ in hm for d in range(1, 6)]
+    want = [c for h in hm for c in h["counts"]]
+    for leaf in (1, 2):
+        assert gpu.perft_batch(ps, ds, leaf_plies=leaf) == want, leaf
+    gpu.tt_init(1024, 0)
+    try:
+        for leaf in (1, 2):
+            gpu.set_leaf_plies(leaf)
+            for h in hm:
+                p = gpu.parse_fen(h["fen"])
+                assert [gpu.perft_bfs(p, d, use_tt=True) for d in range(1, 6)] == h["counts"], (leaf, h["name"])
+            assert gpu.perft_batch(ps, ds, leaf_plies=leaf, use_tt=True) == want, leaf
+    finally:
+        gpu.set_leaf_plies(2)
+        gpu.tt_free()
+
+
 def test_tt_tiny_table_still_exact(gpu):
     gpu.tt_init(1, 0)  # 64 Ki slots: constant overwrites and collisions of slots
     try:
@@ -224,3 +269,61 @@ def test_tt_snapshot_resume_device(gpu, tmp_path):
         assert after[4] > before[4]  # host tier answers from the snapshot
     finally:
         gpu.tt_free()
+
+
+def test_serving_refuses_what_would_share_its_memory(gpu):
+    # while a server exists its slots own the arena and the table: the plain
+    # entry points, table management and a second server are refused with
+    # PG_ERR_BUSY, a full queue refuses new requests, and the served results
+    # stay golden
+    import ctypes
+    L = gpu.lib()
+    rows = [r for r in load_jsonl("c5.jsonl") if "count" in r and r["depth"] >= 5][:24]
+    cfg = gpu.ServeConfig(8 << 30, 512 << 20, 2, 1, 8)  # two slots, at most 8 waiting
+    h = ctypes.c_void_p()
+    assert L.pg_serve_create(ctypes.byref(cfg), ctypes.byref(h)) == 0, gpu.last_error()
+    got, busy = {}, 0
+    try:
+        p = gpu.parse_fen(KIWI)
+        c = ctypes.c_uint64()
+        u = gpu.U128()
+        arr = (gpu.Position * 1)(p)
+        d = (ctypes.c_uint8 * 1)(3)
+        out = (ctypes.c_uint64 * 1)()
+        h2 = ctypes.c_void_p()
+        refused = {
+            "pg_perft_bfs": L.pg_perft_bfs(ctypes.byref(p), 3, 0, None, 0, ctypes.byref(c)),
+            "pg_perft": L.pg_perft(ctypes.byref(p), 3, 0, 0, ctypes.byref(u)),
+            "pg_perft_batch": L.pg_perft_batch(1, arr, d, None, out),
+            "pg_perft_batch_plain": L.pg_perft_batch_plain(1, arr, d, 0, out),
+            "pg_tt_init": L.pg_tt_init(64, 0),
+            "pg_tt_clear": L.pg_tt_clear(),
+            "pg_tt_free": L.pg_tt_free(),
+            "pg_shutdown": L.pg_shutdown(),
+            "pg_serve_create": L.pg_serve_create(ctypes.byref(cfg), ctypes.byref(h2)),
+        }
+        assert all(v == gpu.PG_ERR_BUSY for v in refused.values()), refused
+        want = {r["seed"]: r["count"] for r in rows}
+        pending = list(rows)
+        res = (gpu.ServeResult * 64)()
+        n = ctypes.c_int()
+        deadline = time.time() + 120
+        while len(got) < len(rows) and time.time() < deadline:
+            while pending:
+                r = pending[0]
+                s = L.pg_serve_submit(h, r["seed"], ctypes.byref(gpu.parse_fen(r["fen"])), r["depth"])
+                if s == gpu.PG_ERR_BUSY:
+                    busy += 1
+                    break
+                assert s == 0
+                pending.pop(0)
+            L.pg_serve_poll(h, res, 64, 20, ctypes.byref(n))
+            for i in range(n.value):
+                assert res[i].status == 0
+                got[res[i].id] = res[i].count
+    finally:
+        assert L.pg_serve_destroy(h) == 0
+    assert busy > 0  # 24 requests against 2 slots and 8 places in the queue
+    assert got == want
+    # after the server is gone the plain path runs again
+    assert gpu.perft_bfs(gpu.parse_fen(KIWI), 4) == 4085603

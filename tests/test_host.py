@@ -225,3 +225,93 @@ def test_host_tt_snapshot_resume(engine, tmp_path):
         assert engine.lib().pg_tt_load(snap.encode()) == engine.PG_ERR_INVALID
     finally:
         engine.tt_free()
+
+
+def test_corrupt_snapshot_links_are_refused(engine, tmp_path):
+    # a snapshot whose chain links point outside its entries is refused and
+    # leaves the tables cleared; later counts stay exact
+    import struct
+    p = engine.parse_fen("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -")
+    engine.tt_init(0, 32)
+    try:
+        assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+        snap = tmp_path / "tt.bin"
+        assert engine.lib().pg_tt_save(str(snap).encode()) == 0
+        raw = bytearray(snap.read_bytes())
+        dev_slots, buckets, used = struct.unpack_from("<3Q", raw, 8)
+        assert dev_slots == 0 and used > 2
+        pool = 32 + 4 * buckets
+        for label, off, value in (("head", 32 + 4 * next(i for i in range(buckets)
+                                                           if struct.unpack_from("<I", raw, 32 + 4 * i)[0]), 0xFFFFFFF0),
+                                  ("next", pool + 32 * 1 + 24, used + 5)):
+            bad = bytearray(raw)
+            struct.pack_into("<I", bad, off, value)
+            path = tmp_path / f"bad_{label}.bin"
+            path.write_bytes(bytes(bad))
+            assert engine.lib().pg_tt_load(str(path).encode()) == engine.PG_ERR_INVALID, label
+            assert "outside" in engine.last_error()
+            assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+        assert engine.lib().pg_tt_load(str(snap).encode()) == 0
+    finally:
+        engine.tt_free()
+
+
+def test_invalid_positions_are_refused(engine):
+    # every entry point that takes a caller-built position checks it before
+    # the move generator sees it (a kingless board would index past the
+    # per-square tables)
+    import ctypes
+    good = engine.parse_fen(engine.START_FEN)
+    bad = []
+    p = engine.Position(); p.bb[:] = list(good.bb); p.bb[2] &= ~(1 << 4); bad.append(("no white king", p))  # noqa: E702
+    p = engine.Position(); p.bb[:] = list(good.bb); p.bb[0] |= 1 << 30; bad.append(("black empty square", p))  # noqa: E702
+    p = engine.Position(); p.bb[:] = list(good.bb); p.bb[2] |= 1 << 3; bad.append(("code 7 on d1", p))  # noqa: E702
+    p = engine.Position(); p.bb[:] = list(good.bb); p.color = 2; bad.append(("side 2", p))  # noqa: E702
+    L = engine.lib()
+    n = ctypes.c_int()
+    moves = (ctypes.c_uint16 * 256)()
+    cnt = ctypes.c_uint64()
+    u = engine.U128()
+    for why, p in bad:
+        ref = ctypes.byref(p)
+        assert L.pg_perft_cpu(ref, 3, 0, ctypes.byref(cnt)) == engine.PG_ERR_INVALID, why
+        assert "invalid position" in engine.last_error()
+        assert L.pg_gen_moves(ref, moves, ctypes.byref(n)) == engine.PG_ERR_INVALID, why
+        assert L.pg_count_moves(ref, ctypes.byref(n)) == engine.PG_ERR_INVALID, why
+        assert L.pg_make_move(ref, 0) == engine.PG_ERR_INVALID, why
+        assert L.pg_perft_bfs(ref, 3, 0, None, 0, ctypes.byref(cnt)) == engine.PG_ERR_INVALID, why
+        assert L.pg_perft(ref, 3, 0, 0, ctypes.byref(u)) == engine.PG_ERR_INVALID, why
+        arr = (engine.Position * 2)(good, p)
+        d = (ctypes.c_uint8 * 2)(3, 3)
+        out = (ctypes.c_uint64 * 2)()
+        assert L.pg_perft_batch(2, arr, d, None, out) == engine.PG_ERR_INVALID, why
+        assert L.pg_perft_batch_plain(2, arr, d, 0, out) == engine.PG_ERR_INVALID, why
+    assert engine.perft_cpu(good, 3) == 8902
+
+
+def test_launcher_interface_is_exported(engine):
+    # every function and global that include/perft_launcher.h declares (the
+    # reference's launcher.h names, C++ linkage) is defined in the library,
+    # with the reference's parameter types
+    import subprocess
+    so = os.path.join(ROOT, "perft_gpu_2026_amd", "libperft_gpu.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    funcs = {
+        "initGPU(int)", "initMoveGen()",
+        "estimateLaunchDepth(QuadBitBoard*, GameState*, unsigned char, float*)",
+        "perftLauncher(QuadBitBoard*, GameState*, unsigned char, unsigned int, int)",
+        "perft_gpu_host_bfs(QuadBitBoard*, GameState*, unsigned char, int, void*, unsigned long)",
+        "perftCPU(QuadBitBoard*, GameState*, unsigned char, unsigned int)",
+        "perft_cpu_dispatch(QuadBitBoard*, GameState*, unsigned char, unsigned int)",
+        "initTT(int, int, int, float)", "clearDeviceTTs()", "freeTT()", "printTTStats()", "resetCallStats()",
+        "getOomFallbackCount()", "getEffectiveLD()", "setEffectiveLD(int)",
+        "readFENString(char const*, QuadBitBoard*, GameState*, unsigned char*)",
+    }
+    text = {line.split(" ", 2)[2] for line in out.splitlines() if line.count(" ") >= 2 and " T " in line}
+    assert funcs <= text, funcs - text
+    header = open(os.path.join(ROOT, "include", "perft_launcher.h")).read()
+    for var in ("preAllocatedBufferHost", "g_preAllocatedMemorySize", "g_useTT", "g_lastBfsPeakMemory",
+                "g_lastBfsOom", "g_deviceTTBudgetMB", "g_hostTTBudgetMB"):
+        assert re.search(rf"extern [\w *]+\b{var};", header), var
+        assert re.search(rf" [BD] {var}$", out, flags=re.M), var
+    assert os.path.exists(os.path.join(ROOT, "perft_gpu_2026_amd", "bin", "launcher_caller"))
