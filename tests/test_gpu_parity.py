@@ -378,14 +378,15 @@ def test_reference_launcher_interface(gpu, tables):
     # a C++ program written only against the reference's launcher names
     # (include/perft_launcher.h), linked with the library: the whole depth in
     # one GPU call, a forced OOM reported through g_lastBfsOom, the -cpu
-    # dispatch, and perft.cu's per-depth loop with launch depth 3 (host
-    # recursion above it, with the host tier when the tables are on)
+    # dispatch, and perft.cu's per-depth loop with launch depth 4 (a device
+    # table for the GPU calls, host recursion above them with the host tier,
+    # when the tables are on)
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "bin", "launcher_caller")
     kiwi = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
     want = [48, 2039, 97862, 4085603, 193690690]
-    args = [exe, kiwi, "5", "3"] + ([] if tables else ["-nott"])
+    args = [exe, kiwi, "5", "4"] + ([] if tables else ["-nott"])
     print(f"[gpu] launcher_caller tables={tables}", flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -405,4 +406,5 @@ def test_reference_launcher_interface(gpu, tables):
     assert [int(line.split()[1].rstrip(",")) for line in perft_lines] == want + [want[4]] + want[:3]
     assert counts[5] == want[4]
     assert ("Device TT probes" in r.stdout) == tables
+    assert ("Host TT probes" in r.stdout) == tables
     assert r.stdout.rstrip().endswith("done")
