@@ -174,6 +174,14 @@ int pg_level_makecount(const void *parentBoards, const uint8_t *parentStates, co
  * NULL; *total (host pointer, may be NULL) receives the 64-bit sum
  * (synchronises the stream). */
 int pg_scan_u32(uint32_t *data, uint64_t n, const uint32_t *tileSums, void *scratch, uint64_t *total, void *stream);
+/* No counterpart in the reference: pg_scan_u32 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a 9 MiB static device
+ * array per GPU, held until the process ends; n may be at most 2^28
+ * (268,435,456) entries, larger n returns PG_ERR_INVALID.  Calls no hipMalloc
+ * or hipFree.  Every call on one GPU shares that GPU's workspace, so calls
+ * that overlap on one GPU (other streams or threads) race. */
+int pg_scan_u32_static(uint32_t *data, uint64_t n, const uint32_t *tileSums, uint64_t *total, void *stream);
 /* merge_path_partitions + mp_interval_expand_kernel, perft_kernels.cu:77-173:
  * parentIdx[j] = #{p < n : incl[p] <= j} for j < c.  scratch: 4*(ceil(c/2048)+1) B */
 int pg_expand(const uint32_t *incl, uint64_t n, uint64_t c, uint32_t *parentIdx, void *scratch, void *stream);

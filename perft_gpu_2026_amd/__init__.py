@@ -113,6 +113,7 @@ _SIGS = [
     ("pg_level_makecount", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uint64, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p]),
     ("pg_scan_u32", c_int, [c_void_p, c_uint64, c_void_p, c_void_p, POINTER(c_uint64), c_void_p]),
+    ("pg_scan_u32_static", c_int, [c_void_p, c_uint64, c_void_p, POINTER(c_uint64), c_void_p]),
     ("pg_expand", c_int, [c_void_p, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p]),
     ("pg_genmoves", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_uint64, c_void_p, c_void_p]),
     ("pg_leaf", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uint64, c_int, c_void_p, c_void_p]),

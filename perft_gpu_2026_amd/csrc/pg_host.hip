@@ -270,6 +270,8 @@ int initLib(int device, uint64_t arenaBytes)
     PG_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     PG_HIP(hipMalloc(&L.arena, arenaBytes));
     PG_HIP(hipHostMalloc((void **)&L.hostPinned, 4096, hipHostMallocDefault));
+    // the device address of this GPU's static scan workspace (pg_scan_u32_static)
+    PG_TRY(initStaticScratch(device));
     L.arenaBytes = arenaBytes;
     L.device = device;
     L.init = true;

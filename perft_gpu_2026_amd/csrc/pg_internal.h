@@ -159,6 +159,9 @@ int refuseWhileServing(const char *what);
 int validatePosition(const pg_position *p);
 
 // kernels (pg_kernels.hip)
+// looks up, with `device` current, that device's address of the static scan
+// workspace of pg_scan_u32_static (once per device)
+int initStaticScratch(int device);
 // Blocks of kernel `fn` (block size `blockSize`) resident on the whole device
 // at once: occupancy per CU x CUs.  A persistent grid of exactly this size
 // runs in one round; a larger one leaves a partly filled last round.

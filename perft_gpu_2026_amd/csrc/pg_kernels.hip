@@ -984,6 +984,72 @@ extern "C" int pg_scan_u32(uint32_t *data, uint64_t n, const uint32_t *tileSums,
     return PG_OK;
 }
 
+// ------------------------------------------- scan without a caller workspace
+// The scan's workspace of pg_scan_u32_static lives in this file-scope device
+// array, one copy per GPU, held until the process ends.  Host code never uses
+// the array's name (in host code it is a host-side stand-in, not a device
+// address): initStaticScratch looks up each device's address once, with that
+// device current, and every launch takes that address.
+static constexpr uint64_t STATIC_SCAN_MAX_N = 1ull << 28;
+static constexpr uint64_t STATIC_SCRATCH_BYTES = 9ull << 20;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+static constexpr int MAX_DEVICES = 64;
+static void *g_scratchPtr[MAX_DEVICES];
+
+// bytes of pg_scan_u32's workspace for n entries when the tile sums are
+// computed too: group offsets (G + 1 u64), in-group tile offsets (T u32),
+// group sums (G u32), tile sums (T u32)
+static uint64_t scanScratchBytes(uint64_t n)
+{
+    const u64 T = scanTiles(n), G = scanGroupCount(T);
+    return 8 * (G + 1) + 4 * T + 4 * G + 4 * T;
+}
+static_assert(8 * (((1ull << 28) / 256 + 63) / 64 + 1) + 8 * ((1ull << 28) / 256) +
+                      4 * (((1ull << 28) / 256 + 63) / 64) <= (9ull << 20),
+              "the static scratch holds the workspace of the largest scan it accepts");
+
+namespace pg {
+int initStaticScratch(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) {
+        setError("device %d beyond the static scratch table", device);
+        return PG_ERR_INVALID;
+    }
+    if (g_scratchPtr[device]) return PG_OK;
+    void *p = nullptr;
+    PG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratchPtr[device] = p;
+    return PG_OK;
+}
+}  // namespace pg
+
+extern "C" int pg_scan_u32_static(uint32_t *data, uint64_t n, const uint32_t *tileSums, uint64_t *total,
+                                  void *stream)
+{
+    Lib &L = lib();
+    if (!L.init) return PG_ERR_NOT_INIT;
+    if (n > STATIC_SCAN_MAX_N) {
+        setError("pg_scan_u32_static: n = %llu exceeds the static workspace's maximum of %llu entries",
+                 (unsigned long long)n, (unsigned long long)STATIC_SCAN_MAX_N);
+        return PG_ERR_INVALID;
+    }
+    if (n && !data) {
+        setError("pg_scan_u32_static: bad argument");
+        return PG_ERR_INVALID;
+    }
+    void *ws = (L.device >= 0 && L.device < MAX_DEVICES) ? g_scratchPtr[L.device] : nullptr;
+    if (!ws) {
+        setError("pg_scan_u32_static: no static workspace on device %d", L.device);
+        return PG_ERR_NOT_INIT;
+    }
+    if (scanScratchBytes(n) > STATIC_SCRATCH_BYTES) {
+        setError("pg_scan_u32_static: workspace of %llu bytes exceeds %llu", (unsigned long long)scanScratchBytes(n),
+                 (unsigned long long)STATIC_SCRATCH_BYTES);
+        return PG_ERR_INVALID;
+    }
+    return pg_scan_u32(data, n, tileSums, ws, total, stream);
+}
+
 extern "C" int pg_expand(const uint32_t *incl, uint64_t n, uint64_t c, uint32_t *parentIdx, void *scratch,
                          void *stream)
 {
