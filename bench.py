@@ -76,6 +76,52 @@ def cpu_baseline(fen: str, depth: int, budget_s: float = 15.0) -> dict:
                       f"config-2 position: {nodes} nodes in {dt:.1f} s, single-threaded like the reference -cpu path"}
 
 
+def cpu_configs(pg, threads: int = 16, c3_sample: int = 128) -> dict:
+    """The CPU side of configs 1 and 3, each count checked against golden:
+    config 1 in full (perft 5) on one core, by the oracle and by this engine's
+    own drop-in -cpu path (pg_perft_cpu, the reference's perft_cpu_dispatch);
+    and the first `c3_sample` config-3 positions as independent single-threaded
+    perfts on `threads` cores at once (one position per worker, the way the
+    reference's -cpu path runs a batch: one process per FEN), reported as
+    sum of counts / makespan."""
+    import concurrent.futures as cf
+
+    import oracle
+
+    out = {}
+    c1 = load_fixture("c1.jsonl")[0]
+    want1 = c1["counts"][-1]
+    po, pe = oracle.parse_fen(c1["fen"]), pg.parse_fen(c1["fen"])
+    for kind, fn in (("oracle", lambda: oracle.perft(po, 5, threads=1)), ("engine_cpu", lambda: pg.perft_cpu(pe, 5))):
+        fn()  # warm
+        t0 = time.perf_counter()
+        c = fn()
+        dt = time.perf_counter() - t0
+        assert c == want1, (kind, c, want1)
+        out[f"config1_{kind}"] = {"value": c / dt, "unit": "nodes/s", "cores": 1, "nodes": c, "seconds": dt}
+    # deepest first, so that no worker starts a depth-6 perft at the end
+    rows = sorted(load_fixture("c3.jsonl")[:c3_sample], key=lambda r: -r["depth"])
+    want = [r["count"] for r in rows]
+    eng = [pg.parse_fen(r["fen"]) for r in rows]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:  # ctypes releases the GIL inside each call
+        got = list(ex.map(lambda a: pg.perft_cpu(a[0], a[1]), zip(eng, [r["depth"] for r in rows])))
+    dt_e = time.perf_counter() - t0
+    assert got == want
+    orc = [oracle.parse_fen(r["fen"]) for r in rows]
+    t0 = time.perf_counter()
+    got = oracle.perft_batch(orc, [r["depth"] for r in rows], threads)
+    dt_o = time.perf_counter() - t0
+    assert list(got) == want
+    nodes = sum(want)
+    sample = f"the first {len(rows)} config-3 positions (depth 3-6, {nodes} nodes), one single-threaded perft per worker"
+    out["config3_engine_cpu"] = {"value": nodes / dt_e, "unit": "nodes/s", "cores": threads, "seconds": dt_e,
+                                 "sample": sample}
+    out["config3_oracle"] = {"value": nodes / dt_o, "unit": "nodes/s", "cores": threads, "seconds": dt_o,
+                             "sample": sample}
+    return out
+
+
 # HBM bytes per launch of the default config-2 leaf, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of this bench, each scaled by
 # what the counter reads for a plain 256 MiB-in / 256 MiB-out elementwise
@@ -90,7 +136,7 @@ PMC_TRAFFIC = {"leaf2": 66226.3 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
 # (profiles/leaf2_config2_pmc.md).  Divided by the live launch time, the
 # instruction count gives the issue rate per SIMD and cycle (1,024 SIMDs at
 # the 2.4 GHz peak clock).
-PMC_VALU = {"leaf2": (4.6990e9, 0.906)}
+PMC_VALU = {"leaf2": (4.6990e9, 0.906, 0.1542)}  # + the share of issue slots that issued two (VALU2 / VALU)
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
@@ -98,11 +144,61 @@ def valu_issue(stat: dict, counts: int):
     """counts: the leaf's make + countMoves calls per launch (perft(depth - 1))."""
     if stat["name"] not in PMC_VALU:
         return None
-    insts, lanes = PMC_VALU[stat["name"]]
+    insts, lanes, _ = PMC_VALU[stat["name"]]
     avg_s = stat["ms"] / stat["launches"] * 1e-3
     return {"kernel": stat["name"], "wave_insts_per_launch": insts, "lanes_active": lanes,
             "wave_insts_per_simd_cycle": round(insts / (SIMDS * CLOCK_HZ * avg_s), 4),
             "lane_insts_per_count": round(insts * 64 / counts)}
+
+
+PROBE_KINDS = ["v_xor_b32", "v_bitop3_b32", "v_lshlrev_b64", "v_bcnt_u32_b32", "v_lshl_add_u64", "leaf_mix"]
+
+
+def valu_calibration(pg, iters: int = 100_000) -> dict:
+    """The VALU issue ceiling: the calibration kernel (exactly 32 VALU
+    instructions per loop iteration in 8 independent chains) per instruction
+    form and occupancy, as wave instructions per second over the whole chip and
+    per SIMD and cycle at the 2.4 GHz peak clock.  Each launch runs once warm,
+    then the best of three is kept."""
+    out = {}
+    for kind, name in enumerate(PROBE_KINDS):
+        for waves in (2, 4, 8):
+            pg.valu_probe(kind, waves, iters // 10)
+            best = min(pg.valu_probe(kind, waves, iters)[0] for _ in range(3))
+            insts = pg.valu_probe(kind, waves, 1)[1] * iters
+            rate = insts / (best * 1e-3)
+            out[f"{name}_w{waves}"] = {"wave_insts_per_s": rate, "ms": round(best, 4),
+                                       "wave_insts_per_simd_cycle": round(rate / (SIMDS * CLOCK_HZ), 4)}
+    return out
+
+
+def calibrated_issue_ceiling(calib: dict) -> float:
+    """The largest share of VALU issue cycles the calibration kernel fills
+    with instructions that never issue two per cycle (64-bit shift, popcount,
+    64-bit add), at eight waves per SIMD: what a saturating instruction stream
+    reaches with this clock and launch shape."""
+    return max(calib[f"{PROBE_KINDS[k]}_w8"]["wave_insts_per_s"] for k in (2, 3, 4)) / (SIMDS * CLOCK_HZ / 4)
+
+
+def roofline_valu(stat: dict, hbm: dict, ceiling: float) -> dict:
+    """The leaf against its VALU ceiling.  A SIMD issues one VALU instruction
+    per four-cycle issue slot (1,024 SIMDs x 2.4 GHz / 4 = 614.4 G slots/s),
+    and some 32-bit forms of two waves issue together in one slot
+    (SQ_ACTIVE_INST_VALU2).  achieved = the issue slots the leaf fills per
+    second: its PMC VALU instructions per launch, less the second
+    instructions of dual-issued slots, over the live launch time."""
+    insts, lanes, dual = PMC_VALU[stat["name"]]
+    avg_s = stat["ms"] / stat["launches"] * 1e-3
+    peak = SIMDS * CLOCK_HZ / 4 / 1e9
+    achieved = insts * (1 - dual) / avg_s / 1e9
+    return {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 2), "unit": "G issue slots/s",
+            "frac": round(achieved / peak, 4), "traffic": hbm["traffic"], "kernel": stat["name"],
+            "avg_launch_ms": round(avg_s * 1e3, 4), "wave_insts_per_launch": insts, "dual_issue_share": dual,
+            "lanes_active": lanes, "wave_insts_per_simd_cycle": round(insts / (SIMDS * CLOCK_HZ * avg_s), 4),
+            "calibrated_ceiling_frac": round(ceiling, 4),
+            "op": "one four-cycle VALU issue slot of one SIMD",
+            "hbm": {k: hbm[k] for k in ("achieved", "peak", "unit", "frac")},
+            "algorithmic_bytes_per_launch": hbm["algorithmic_bytes_per_launch"]}
 
 
 def roofline(stat: dict, traffic=None) -> dict:
@@ -309,15 +405,26 @@ def serve_leg(pg, n_req: int, rate: float, tt_gb: int, slots: int) -> dict:
         t += rnd.expovariate(lam)
         arrivals.append(t)
     done = run(arrivals)
-    bad = [k for k, v in done.items() if v[4] != 0 or (k in gold and v[0] != gold[k])]
     lat = sorted(v[3] - v[1] for v in done.values())
     nodes = sum(v[0] for v in done.values())
     span = max(v[3] for v in done.values()) - min(v[1] for v in done.values())
+    # every served count is checked: against the oracle's golden count where
+    # the fixture has one (depth <= 5 and 64 of depth 6), the rest against the
+    # batched engine without the table, after the server is gone
+    rest = [(rid, p, d) for rid, p, d in reqs if rid not in gold]
+    print(f"[bench] serving: checking {len(rest)} results against the batched engine without the table",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    plain = dict(zip([r[0] for r in rest], pg.perft_batch([r[1] for r in rest], [r[2] for r in rest])))
+    t_check = time.perf_counter() - t0
+    want = {**gold, **plain}
+    bad = [k for k, v in done.items() if v[4] != 0 or v[0] != want[k]]
     return {"workload": f"config-5: first {len(reqs)} requests (0-60 plies, depth 3-7), TT shared "
                         f"({tt_gb} GB), 200 GB budget, {slots} slots, Poisson arrivals",
             "rate_req_per_s": lam, "capacity_req_per_s": capacity, "p50_ms": 1e3 * lat[len(lat) // 2],
             "p99_ms": 1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], "nodes_per_s": nodes / span,
-            "golden_checked": sum(1 for k in done if k in gold), "mismatches": len(bad)}
+            "golden_checked": sum(1 for k in done if k in gold), "checked_without_table": len(plain),
+            "check_s": t_check, "checked": len(done), "mismatches": len(bad) + (len(reqs) - len(done))}
 
 
 def copy_probe(pg) -> None:
@@ -358,6 +465,7 @@ def main():
     ap.add_argument("--serve-tt-gb", type=int, default=16,
                     help="shared transposition table of the serving leg, inside its 200 GB budget")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
+    ap.add_argument("--valu-probe", action="store_true", help="only run the VALU calibration kernel and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the engine runs on one GPU and has no multi-GPU mode "
@@ -370,6 +478,10 @@ def main():
     if a.copy_probe:
         pg.init(arena_bytes=1 << 30)
         copy_probe(pg)
+        return
+    if a.valu_probe:
+        pg.init(arena_bytes=1 << 30)
+        print(json.dumps(valu_calibration(pg)), flush=True)
         return
     fx = load_fixture("c2.jsonl")[0]
     depth, want = fx["depth"], fx["counts"][-1]
@@ -405,6 +517,15 @@ def main():
     # the HBM-bound level kernels (make+count and genmoves are VALU-bound)
     levels = [v for k, v in stats.items() if k in ("scan", "expand")]
     top_level = max(levels, key=lambda v: v["ms"]) if levels else None
+    print("[bench] VALU calibration", file=sys.stderr, flush=True)
+    calib = valu_calibration(pg)
+    leaf_roofline = None
+    if leaf:
+        hbm = roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None)
+        if leaf["name"] in PMC_VALU and a.leaf2_variant == 0:
+            leaf_roofline = roofline_valu(leaf, hbm, calibrated_issue_ceiling(calib))
+        else:
+            leaf_roofline = dict(hbm, note="no PMC instruction count for this leaf form: HBM bytes only")
     out = {
         "metric": "perft nodes/s on 1 MI355X, batched vs plain BFS; p50/p99 query latency",
         "value": value,
@@ -420,11 +541,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
                    "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
-        "roofline": dict(roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None),
-                         note="the leaf is integer VALU work with no contraction: it is bound by its VALU "
-                              "instruction stream (valu_issue), so its HBM fraction is small by nature") if leaf else None,
+        "roofline": leaf_roofline,
         "roofline_levels": roofline(top_level) if top_level else None,
         "valu_issue": valu_issue(leaf, fx["counts"][depth - 2]) if leaf and a.leaf2_variant == 0 else None,
+        "valu_calibration": calib,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}
                     for k, v in stats.items()},
@@ -447,6 +567,7 @@ def main():
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(fx["fen"], depth)
+        out["cpu_baseline"].update(cpu_configs(pg, threads=min(16, os.cpu_count() or 1)))
     print(json.dumps(out))
 
 

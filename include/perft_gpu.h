@@ -210,6 +210,14 @@ typedef struct pg_kernel_stat {
     double   big_ms;
     uint64_t big_launches;
 } pg_kernel_stat;
+/* VALU calibration (no reference counterpart): a kernel whose loop is exactly
+ * 32 VALU instructions per iteration in 8 independent chains, on wavesPerSimd
+ * (1-8) waves per SIMD of every CU, `iters` iterations.  kind 0 v_xor_b32,
+ * 1 v_bitop3_b32, 2 v_lshlrev_b64, 3 v_bcnt_u32_b32, 4 v_lshl_add_u64, 5 the
+ * move counter's mix (bitwise, 64-bit shift, select, add, popcount).  *ms is
+ * the launch's time, *waveInsts the VALU instructions it issued: their
+ * quotient is the VALU issue rate the leaf kernels are measured against. */
+int pg_valu_probe(int kind, int wavesPerSimd, uint64_t iters, double *ms, uint64_t *waveInsts);
 int pg_profile_enable(int on);
 int pg_profile_reset(void);
 int pg_profile_get(pg_kernel_stat *out, int maxOut, int *nOut);

@@ -124,6 +124,7 @@ _SIGS = [
     ("pg_serve_poll", c_int, [c_void_p, POINTER(ServeResult), c_int, c_int, POINTER(c_int)]),
     ("pg_serve_destroy", c_int, [c_void_p]),
     ("pg_clock", c_double, []),
+    ("pg_valu_probe", c_int, [c_int, c_int, c_uint64, POINTER(c_double), POINTER(c_uint64)]),
     ("pg_profile_enable", c_int, [c_int]),
     ("pg_profile_reset", c_int, []),
     ("pg_profile_get", c_int, [POINTER(KernelStat), c_int, POINTER(c_int)]),
@@ -309,6 +310,13 @@ def parse_cli_counts(stdout: str) -> dict[int, int]:
             d = int(line[6:8])
             out[d] = int(line.split(":")[1].split(",")[0])
     return out
+
+
+def valu_probe(kind: int, waves_per_simd: int, iters: int) -> tuple[float, int]:
+    """(milliseconds, wave instructions) of one launch of the VALU calibration kernel."""
+    ms, insts = ctypes.c_double(), c_uint64()
+    _check(lib().pg_valu_probe(kind, waves_per_simd, iters, ctypes.byref(ms), ctypes.byref(insts)), "pg_valu_probe")
+    return ms.value, insts.value
 
 
 def profile_enable(on: bool = True) -> None:

@@ -32,7 +32,7 @@ def test_roofline_fields():
 
 
 def test_valu_issue_arithmetic():
-    insts, lanes = bench.PMC_VALU["leaf2"]
+    insts, lanes, _ = bench.PMC_VALU["leaf2"]
     # one launch lasting exactly insts / (SIMDs x clock) seconds issues one
     # wave instruction per SIMD and cycle
     ms = insts / (bench.SIMDS * bench.CLOCK_HZ) * 1e3
@@ -41,3 +41,23 @@ def test_valu_issue_arithmetic():
     assert v["lane_insts_per_count"] == pytest.approx(700, abs=1)
     assert v["lanes_active"] == lanes
     assert bench.valu_issue({"name": "leaf1", "launches": 1, "ms": 1.0}, counts=1) is None
+
+
+def test_valu_roofline_arithmetic():
+    # a leaf launch whose single-issued slots fill every SIMD issue slot of the
+    # launch time sits at frac 1; the calibrated ceiling is the best
+    # single-issue probe's share of the same slots
+    insts, lanes, dual = bench.PMC_VALU["leaf2"]
+    slots_per_s = bench.SIMDS * bench.CLOCK_HZ / 4
+    ms = insts * (1 - dual) / slots_per_s * 1e3
+    stat = {"name": "leaf2", "launches": 2, "ms": 2 * ms, "bytes": 2 * 1e8}
+    hbm = bench.roofline(stat, traffic=1.4e8)
+    calib = {f"{n}_w8": {"wave_insts_per_s": 0.9 * slots_per_s} for n in bench.PROBE_KINDS}
+    calib["v_bcnt_u32_b32_w8"] = {"wave_insts_per_s": 0.95 * slots_per_s}
+    ceiling = bench.calibrated_issue_ceiling(calib)
+    assert ceiling == pytest.approx(0.95)
+    r = bench.roofline_valu(stat, hbm, ceiling)
+    assert r["bound"] == "valu" and r["frac"] == pytest.approx(1.0, rel=1e-3)
+    assert r["peak"] == pytest.approx(614.4) and r["achieved"] == pytest.approx(614.4, rel=1e-3)
+    assert r["traffic"] == 1.4e8 and r["hbm"]["frac"] == hbm["frac"]
+    assert r["lanes_active"] == lanes and r["calibrated_ceiling_frac"] == pytest.approx(0.95)
