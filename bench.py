@@ -466,6 +466,8 @@ def main():
                     help="shared transposition table of the serving leg, inside its 200 GB budget")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
     ap.add_argument("--valu-probe", action="store_true", help="only run the VALU calibration kernel and exit")
+    ap.add_argument("--slider", type=int, default=0,
+                    help="slider attacks of the fused leaf: 0 fills (default), 1 magic tables, 2 magic with LDS bishops")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: the engine runs on one GPU and has no multi-GPU mode "
@@ -490,6 +492,9 @@ def main():
     if a.leaf:
         pg.set_leaf_plies(a.leaf)
     pg.lib().pg_set_leaf2_variant(a.leaf2_variant)
+    if a.slider and pg.lib().pg_set_slider_attacks(a.slider) != 0:
+        raise SystemExit(f"pg_set_slider_attacks({a.slider}): {pg.last_error()}")
+    pmc_known = a.leaf2_variant == 0 and a.slider == 0  # the PMC figures are of the default leaf
     if a.leaf_grid:
         pg.lib().pg_set_leaf_grid(a.leaf_grid)
     pos = pg.parse_fen(fx["fen"])
@@ -521,8 +526,8 @@ def main():
     calib = valu_calibration(pg)
     leaf_roofline = None
     if leaf:
-        hbm = roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if a.leaf2_variant == 0 else None)
-        if leaf["name"] in PMC_VALU and a.leaf2_variant == 0:
+        hbm = roofline(leaf, PMC_TRAFFIC.get(leaf["name"]) if pmc_known else None)
+        if leaf["name"] in PMC_VALU and pmc_known:
             leaf_roofline = roofline_valu(leaf, hbm, calibrated_issue_ceiling(calib))
         else:
             leaf_roofline = dict(hbm, note="no PMC instruction count for this leaf form: HBM bytes only")
@@ -540,10 +545,11 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": "config-2: startpos + 8 random plies (seed 2), perft 7, no TT, one plain BFS call",
-                   "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2},
+                   "fen": fx["fen"], "depth": depth, "nodes_per_step": want, "leaf_plies": a.leaf or 2,
+                   "slider_attacks": ["fills", "magic", "magic_lds_bishops"][a.slider]},
         "roofline": leaf_roofline,
         "roofline_levels": roofline(top_level) if top_level else None,
-        "valu_issue": valu_issue(leaf, fx["counts"][depth - 2]) if leaf and a.leaf2_variant == 0 else None,
+        "valu_issue": valu_issue(leaf, fx["counts"][depth - 2]) if leaf and pmc_known else None,
         "valu_calibration": calib,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] else None}

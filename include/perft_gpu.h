@@ -136,6 +136,17 @@ void pg_set_leaf2_variant(int variant);
  * leaf uses twice as many) -- a tuning knob of this engine, no reference
  * counterpart */
 void pg_set_leaf_grid(int blocksPerCU);
+/* slider attacks of the fused two-ply leaf's move counter: 0 = computed
+ * fills (default), 1 = fancy magic lookups (the reference's method,
+ * MoveGeneratorBitboard.h:589-648) with the tables in device memory, 2 = the
+ * same with the bishop table in each block's LDS.  The first call with 1 or 2
+ * builds the tables (own multipliers, checked for every blocker subset) and
+ * uploads them; needs pg_init. */
+int pg_set_slider_attacks(int mode);
+/* host-only check of the magic tables (no GPU needed): builds them and looks
+ * up `samples` random occupancies per square and slider against a ray walk;
+ * *tries = multipliers tried by the search.  PG_OK or PG_ERR_INVALID. */
+int pg_magic_check(uint64_t samples, uint64_t *tries);
 /* decimal text of a 128-bit count (uint128.h:46-108) */
 int pg_u128_to_string(uint64_t lo, uint64_t hi, char *buf, int n);
 

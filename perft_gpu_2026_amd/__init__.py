@@ -103,6 +103,8 @@ _SIGS = [
     ("pg_set_leaf_plies", None, [c_int]),
     ("pg_set_leaf2_variant", None, [c_int]),
     ("pg_set_leaf_grid", None, [c_int]),
+    ("pg_set_slider_attacks", c_int, [c_int]),
+    ("pg_magic_check", c_int, [c_uint64, POINTER(c_uint64)]),
     ("pg_u128_to_string", c_int, [c_uint64, c_uint64, c_char_p, c_int]),
     ("pg_tt_init", c_int, [c_uint64, c_uint64]),
     ("pg_tt_clear", c_int, []),

@@ -33,6 +33,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "pg_magic.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PG_HD __host__ __device__ __forceinline__
@@ -466,8 +468,27 @@ PG_HD BB epSources(const Frame &f, u8 state)
 // only when the side to move is in check from a pawn that just double-pushed
 // between two of its pawns (221,239 times among the perft(10) leaves of
 // Position 3, whose published count includes it).
-template <int C, bool EXACT>
-PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state)
+// Slider attacks by table lookup (pg_magic.h): `sq` holds the 64 rook rows
+// then the 64 bishop rows, `ratt` / `batt` the rook and bishop attack sets
+// (device memory, or a block's LDS copy of the bishop sets).
+struct MagicSliders {
+    const MagicSquare *sq;
+    const u64 *ratt, *batt;
+    PG_HD BB rook(int s, BB occ) const
+    {
+        const MagicSquare &m = sq[s];
+        return BB(ratt[m.offset + (((occ.v() & m.mask) * m.magic) >> m.shift)]);
+    }
+    PG_HD BB bishop(int s, BB occ) const
+    {
+        const MagicSquare &m = sq[64 + s];
+        return BB(batt[m.offset + (((occ.v() & m.mask) * m.magic) >> m.shift)]);
+    }
+};
+struct FillSliders {};  // the computed fills below (default)
+
+template <int C, bool EXACT, class SL = FillSliders>
+PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state, const SL &sl = SL())
 {
     Frame f = framePins<C>(q);
     const Sets &s = f.s;
@@ -475,29 +496,52 @@ PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state)
     BB targets = ~s.us & f.checkMask;
     BB freeP = ~f.pinned;
 
-    // sliders: a direction's fill reaches each target from exactly one slider,
-    // so per-direction popcounts count moves exactly; a pinned slider moves
-    // only along its pin line
-    BB D = s.diag & s.us, O = s.orth & s.us;
-    BB eD = s.diag & s.them, eO = s.orth & s.them;
-    BB dA1H8 = D & (freeP | f.pinDiag), dA8H1 = D & (freeP | f.pinAnti);
-    BB oFile = O & (freeP | f.pinFile), oRank = O & (freeP | f.pinRank);
-    BB att = f.att, r;
+    BB att = f.att;
     u32 n = 0;
-    rayPair<8, ~0ULL>(eO, oFile, empty, att, r);
-    n += popc(r & targets);
-    rayPair<-8, ~0ULL>(eO, oFile, empty, att, r);
-    n += popc(r & targets);
-    att |= eastAttacks(eO, s.occ) | westAttacks(eO, s.occ);
-    n += popc(eastAttacks(oRank, s.occ) & targets) + popc(westAttacks(oRank, s.occ) & targets);
-    rayPair<9, NOT_A>(eD, dA1H8, empty, att, r);
-    n += popc(r & targets);
-    rayPair<-9, NOT_H>(eD, dA1H8, empty, att, r);
-    n += popc(r & targets);
-    rayPair<7, NOT_H>(eD, dA8H1, empty, att, r);
-    n += popc(r & targets);
-    rayPair<-7, NOT_A>(eD, dA8H1, empty, att, r);
-    n += popc(r & targets);
+    if constexpr (__is_same(SL, MagicSliders)) {
+        // one lookup per slider: the enemy's through our king (its squares
+        // behind the king are attacked), ours limited to a pin line when pinned
+        const BB occNoK = s.occ ^ f.kBit;
+        for (BB b = s.diag & s.them; b; b = clearLow(b)) att |= sl.bishop(lsb(b), occNoK);
+        for (BB b = s.orth & s.them; b; b = clearLow(b)) att |= sl.rook(lsb(b), occNoK);
+        for (BB b = s.diag & s.us & ~(f.pinFile | f.pinRank); b; b = clearLow(b)) {
+            const int from = lsb(b);
+            const BB fb = bit(from);
+            BB t = sl.bishop(from, s.occ) & targets;
+            t &= (fb & f.pinDiag) ? f.pinDiag : ((fb & f.pinAnti) ? f.pinAnti : BB(~0ULL));
+            n += popc(t);
+        }
+        for (BB b = s.orth & s.us & ~(f.pinDiag | f.pinAnti); b; b = clearLow(b)) {
+            const int from = lsb(b);
+            const BB fb = bit(from);
+            BB t = sl.rook(from, s.occ) & targets;
+            t &= (fb & f.pinFile) ? f.pinFile : ((fb & f.pinRank) ? f.pinRank : BB(~0ULL));
+            n += popc(t);
+        }
+    } else {
+        // sliders: a direction's fill reaches each target from exactly one
+        // slider, so per-direction popcounts count moves exactly; a pinned
+        // slider moves only along its pin line
+        BB D = s.diag & s.us, O = s.orth & s.us;
+        BB eD = s.diag & s.them, eO = s.orth & s.them;
+        BB dA1H8 = D & (freeP | f.pinDiag), dA8H1 = D & (freeP | f.pinAnti);
+        BB oFile = O & (freeP | f.pinFile), oRank = O & (freeP | f.pinRank);
+        BB r;
+        rayPair<8, ~0ULL>(eO, oFile, empty, att, r);
+        n += popc(r & targets);
+        rayPair<-8, ~0ULL>(eO, oFile, empty, att, r);
+        n += popc(r & targets);
+        att |= eastAttacks(eO, s.occ) | westAttacks(eO, s.occ);
+        n += popc(eastAttacks(oRank, s.occ) & targets) + popc(westAttacks(oRank, s.occ) & targets);
+        rayPair<9, NOT_A>(eD, dA1H8, empty, att, r);
+        n += popc(r & targets);
+        rayPair<-9, NOT_H>(eD, dA1H8, empty, att, r);
+        n += popc(r & targets);
+        rayPair<7, NOT_H>(eD, dA8H1, empty, att, r);
+        n += popc(r & targets);
+        rayPair<-7, NOT_A>(eD, dA8H1, empty, att, r);
+        n += popc(r & targets);
+    }
 
     n += popc(f.kAtt & ~s.us & ~att);
 
@@ -546,6 +590,18 @@ template <int C>
 PG_HD u32 countChildren(const QuadBitBoard &q, u8 state)
 {
     return countMovesT<C, true>(q, state);
+}
+
+// the same two counts with a chosen slider-attack method (FillSliders or MagicSliders)
+template <int C, class SL>
+PG_HD u32 countMovesSL(const QuadBitBoard &q, u8 state, const SL &sl)
+{
+    return countMovesT<C, false, SL>(q, state, sl);
+}
+template <int C, class SL>
+PG_HD u32 countChildrenSL(const QuadBitBoard &q, u8 state, const SL &sl)
+{
+    return countMovesT<C, true, SL>(q, state, sl);
 }
 
 PG_HD u32 countMoves(const QuadBitBoard &q, u8 state, int color)

@@ -21,6 +21,8 @@
 // PG_ERR_ARENA_OOM and the caller splits one ply down), sizes are 64-bit.
 #include <string.h>
 
+#include <memory>
+#include <type_traits>
 #include <unordered_map>
 
 #include "pg_internal.h"
@@ -211,10 +213,23 @@ constexpr int L2_CHUNK = 64;
 constexpr int L2_CAP = 2016;
 constexpr int L2_WAVES = 4;
 
+// Fancy magic slider tables (pg_magic.h) for the table-lookup form of the
+// fused leaf's counter: square rows and attack sets in device memory (845
+// KB, L2-resident), filled once by uploadMagicTables.
+static __device__ MagicSquare g_magicSq[128];
+static __device__ u64 g_magicAtt[MAGIC_ENTRIES];
+
+// Slider-attack method of the fused leaf: computed fills (the default), table
+// lookups from device memory, or lookups with the bishop sets and the square
+// rows copied into each block's LDS (46 KB more per block: two blocks per CU
+// instead of four)
+enum { SL_FILLS = 0, SL_MAGIC = 1, SL_MAGIC_LDS = 2 };
+int g_sliderMode = SL_FILLS;
+
 // Chunks are taken either by a static stride over the grid, or (QUEUE) from
 // an atomic counter `work` so that no wave idles while others still hold
 // several chunks (the counter starts at zero; the grid is the resident one).
-template <int C, int CHUNK, int CAP, bool QUEUE>
+template <int C, int CHUNK, int CAP, bool QUEUE, int SLM = SL_FILLS>
 __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, const u8 *__restrict__ ps,
                                           const u32 *__restrict__ pidx, const u16 *__restrict__ mv, u64 n,
                                           u64 *__restrict__ sum, u64 *work)
@@ -225,6 +240,18 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
     __shared__ u16 slm[L2_WAVES][CAP];
     __shared__ u8 sll[L2_WAVES][CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    using SL = typename std::conditional<SLM == SL_FILLS, FillSliders, MagicSliders>::type;
+    SL sl{};
+    if constexpr (SLM == SL_MAGIC) {
+        sl = MagicSliders{g_magicSq, g_magicAtt, g_magicAtt + MAGIC_ROOK_ENTRIES};
+    } else if constexpr (SLM == SL_MAGIC_LDS) {
+        __shared__ u64 lb[MAGIC_BISHOP_ENTRIES];
+        __shared__ MagicSquare lsq[128];
+        for (int i = threadIdx.x; i < MAGIC_BISHOP_ENTRIES; i += blockDim.x) lb[i] = g_magicAtt[MAGIC_ROOK_ENTRIES + i];
+        for (int i = threadIdx.x; i < 128; i += blockDim.x) lsq[i] = g_magicSq[i];
+        __syncthreads();
+        sl = MagicSliders{lsq, g_magicAtt, lb};
+    }
     u64 acc = 0;
     const u64 stride = (u64)gridDim.x * L2_WAVES * CHUNK;
     u64 next = ((u64)blockIdx.x * L2_WAVES + w) * CHUNK;
@@ -248,7 +275,7 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
             makeMove<C>(q, st, mv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            cnt = countChildren<1 - C>(q, st);
+            cnt = countChildrenSL<1 - C>(q, st, sl);
         }
         u32 incl = waveInclScan(cnt);
         u32 off = incl - cnt;
@@ -273,13 +300,24 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
                 makeMove<1 - C>(c, s2, x);
-                acc += countMoves<C>(c, s2);
+                acc += countMovesSL<C>(c, s2, sl);
             }
             waveLdsSync();
         }
     }
     u64 s = waveSum64(acc);
     if (lane == 0 && s) atomicAdd((unsigned long long *)sum, (unsigned long long)s);
+}
+
+// the queue-fed leaf with the table-lookup counter (pg_set_slider_attacks)
+template <int C, int SLM>
+__global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2q_magic(const QuadBitBoard *__restrict__ pb,
+                                                                const u8 *__restrict__ ps,
+                                                                const u32 *__restrict__ pidx,
+                                                                const u16 *__restrict__ mv, u64 n,
+                                                                u64 *__restrict__ sum, u64 *work)
+{
+    leaf2Body<C, L2_CHUNK, L2_CAP, true, SLM>(pb, ps, pidx, mv, n, sum, work);
 }
 
 // Two shapes of the same leaf: 64-entry chunks (default) and 32-entry chunks
@@ -662,6 +700,21 @@ static int launchLeaf(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx, con
         if (g_leaf2Variant == 1) {
             if (color == WHITE) hipLaunchKernelGGL(k_leaf2_c32<WHITE>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
             else hipLaunchKernelGGL(k_leaf2_c32<BLACK>, g, b, 0, st, pb, ps, pidx, mv, n, sum);
+        } else if (work && g_sliderMode != SL_FILLS) {
+            const void *fn = g_sliderMode == SL_MAGIC
+                                 ? (color == WHITE ? (const void *)k_leaf2q_magic<WHITE, SL_MAGIC>
+                                                   : (const void *)k_leaf2q_magic<BLACK, SL_MAGIC>)
+                                 : (color == WHITE ? (const void *)k_leaf2q_magic<WHITE, SL_MAGIC_LDS>
+                                                   : (const void *)k_leaf2q_magic<BLACK, SL_MAGIC_LDS>);
+            u64 r = residentGrid(fn, L2_WAVES * 64);
+            dim3 gq((unsigned)(blocks < r ? blocks : r));
+            if (g_sliderMode == SL_MAGIC) {
+                if (color == WHITE) hipLaunchKernelGGL((k_leaf2q_magic<WHITE, SL_MAGIC>), gq, b, 0, st, pb, ps, pidx, mv, n, sum, work);
+                else hipLaunchKernelGGL((k_leaf2q_magic<BLACK, SL_MAGIC>), gq, b, 0, st, pb, ps, pidx, mv, n, sum, work);
+            } else {
+                if (color == WHITE) hipLaunchKernelGGL((k_leaf2q_magic<WHITE, SL_MAGIC_LDS>), gq, b, 0, st, pb, ps, pidx, mv, n, sum, work);
+                else hipLaunchKernelGGL((k_leaf2q_magic<BLACK, SL_MAGIC_LDS>), gq, b, 0, st, pb, ps, pidx, mv, n, sum, work);
+            }
         } else if (work) {
             u64 r = residentGrid((const void *)(color == WHITE ? k_leaf2q<WHITE> : k_leaf2q<BLACK>), L2_WAVES * 64);
             dim3 gq((unsigned)(blocks < r ? blocks : r));
@@ -1154,3 +1207,32 @@ extern "C" int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind)
 
 extern "C" void pg_set_leaf2_variant(int v) { g_leaf2Variant = v == 1 ? 1 : 0; }
 extern "C" void pg_set_leaf_grid(int blocksPerCU) { g_leafBlocksPerCU = blocksPerCU >= 1 && blocksPerCU <= 256 ? blocksPerCU : 16; }
+
+// slider-attack method of the fused two-ply leaf: 0 = computed fills (the
+// default), 1 = fancy magic tables in device memory, 2 = the same with the
+// bishop sets and the square rows in each block's LDS.  The tables are built
+// (and checked against a ray walk for every blocker subset) and uploaded on
+// the first call that asks for them.
+extern "C" int pg_set_slider_attacks(int mode)
+{
+    if (mode < SL_FILLS || mode > SL_MAGIC_LDS) {
+        setError("pg_set_slider_attacks: mode %d is not 0, 1 or 2", mode);
+        return PG_ERR_INVALID;
+    }
+    if (mode != SL_FILLS) {
+        static bool uploaded = false;
+        if (!lib().init) return PG_ERR_NOT_INIT;
+        if (!uploaded) {
+            std::unique_ptr<MagicTables> t(new MagicTables);
+            if (!buildMagicTables(*t)) {
+                setError("pg_set_slider_attacks: the magic search failed");
+                return PG_ERR_INVALID;
+            }
+            PG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_magicSq), t->sq, sizeof t->sq));
+            PG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_magicAtt), t->att, sizeof t->att));
+            uploaded = true;
+        }
+    }
+    g_sliderMode = mode;
+    return PG_OK;
+}

@@ -408,3 +408,22 @@ def test_reference_launcher_interface(gpu, tables):
     assert ("Device TT probes" in r.stdout) == tables
     assert ("Host TT probes" in r.stdout) == tables
     assert r.stdout.rstrip().endswith("done")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_magic_table_leaf(gpu, mode):
+    # the fused leaf with fancy magic lookups (tables in device memory, or the
+    # bishop table in LDS) gives the golden counts: the published positions,
+    # the en-passant-evasion and high-mobility positions and config 2
+    assert gpu.lib().pg_set_slider_attacks(mode) == 0, gpu.last_error()
+    try:
+        for name, fen, d, c in _known(2_000_000_000):
+            print(f"[gpu] magic leaf mode={mode} {name} d={d}", flush=True)
+            assert gpu.perft_bfs(gpu.parse_fen(fen), d) == c, (name, d)
+        for p in load_json("ep_evasion.json")["positions"] + load_json("high_mobility.json")["positions"]:
+            pos = gpu.parse_fen(p["fen"])
+            assert [gpu.perft_bfs(pos, d) for d in range(1, len(p["counts"]) + 1)] == p["counts"], p["name"]
+        r = load_jsonl("c2.jsonl")[0]
+        assert gpu.perft_bfs(gpu.parse_fen(r["fen"]), 7) == r["counts"][6]
+    finally:
+        assert gpu.lib().pg_set_slider_attacks(0) == 0

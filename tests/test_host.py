@@ -315,3 +315,14 @@ def test_launcher_interface_is_exported(engine):
         assert re.search(rf"extern [\w *]+\b{var};", header), var
         assert re.search(rf" [BD] {var}$", out, flags=re.M), var
     assert os.path.exists(os.path.join(ROOT, "perft_gpu_2026_amd", "bin", "launcher_caller"))
+
+
+def test_magic_tables_match_ray_walk(engine):
+    # the fancy magic tables of the table-lookup leaf: the builder checks
+    # every blocker subset of every square as it fills them; here 2,000 random
+    # full-board occupancies per square and slider go through the finished
+    # tables against a ray walk
+    import ctypes
+    tries = ctypes.c_uint64()
+    assert engine.lib().pg_magic_check(2000, ctypes.byref(tries)) == 0
+    assert tries.value > 128
