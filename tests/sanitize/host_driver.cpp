@@ -28,5 +28,8 @@ int main() {
   for (uint64_t s = 0; s < 300; s++) { pg_position q; int pl, d; pg_random_request(s, 0, 80, 3, 6, &q, &pl, &d); char b[128]; pg_write_fen(&q, b, 128); }
   char b[64]; pg_u128_to_string(~0ULL, ~0ULL, b, 64); if (strcmp(b, "340282366920938463463374607431768211455")) bad++;
   pg_u128_to_string(5, 0, b, 2);  // truncation path
+  uint64_t tries = 0; if (pg_magic_check(100, &tries) != 0) bad++;  // magic table search and lookups
+  pg_position k; pg_parse_fen(fens[0], &k); k.bb[2] &= ~(1ULL << 4);  // no white king
+  if (pg_perft_cpu(&k, 3, 0, &c1) != PG_ERR_INVALID) bad++;
   printf("asan driver: %s\n", bad ? "FAIL" : "ok"); return bad;
 }

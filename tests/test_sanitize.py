@@ -24,7 +24,7 @@ def test_host_code_under_asan_ubsan(tmp_path):
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
     objs = []
     procs = []
-    for f in ("pg_kernels", "pg_host", "pg_tt", "pg_batch", "pg_serve"):
+    for f in ("pg_kernels", "pg_host", "pg_tt", "pg_batch", "pg_serve", "pg_launcher_api", "pg_probe", "pg_magic"):
         o = str(tmp_path / f"{f}.o")
         objs.append(o)
         procs.append(subprocess.Popen([HIPCC] + flags + ["-c", os.path.join(src, f + ".hip"), "-o", o],
