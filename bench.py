@@ -128,7 +128,7 @@ def cpu_configs(pg, threads: int = 16, c3_sample: int = 128) -> dict:
 # kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0);
 # the writes are mostly the chunk queue's atomics.
 # Commands and figures: profiles/leaf2_config2_pmc.md.
-PMC_TRAFFIC = {"leaf2": 66226.3 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
+PMC_TRAFFIC = {"leaf2": 66204.9 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
 
 # VALU work per launch of the same leaf: SQ_INSTS_VALU (wave64 instructions)
 # and SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64), the share of lanes
@@ -136,7 +136,7 @@ PMC_TRAFFIC = {"leaf2": 66226.3 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
 # (profiles/leaf2_config2_pmc.md).  Divided by the live launch time, the
 # instruction count gives the issue rate per SIMD and cycle (1,024 SIMDs at
 # the 2.4 GHz peak clock).
-PMC_VALU = {"leaf2": (4.6990e9, 0.906, 0.1542)}  # + the share of issue slots that issued two (VALU2 / VALU)
+PMC_VALU = {"leaf2": (4.5917e9, 0.904, 0.1783)}  # + the share of issue slots that issued two (VALU2 / VALU)
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
@@ -151,7 +151,10 @@ def valu_issue(stat: dict, counts: int):
             "lane_insts_per_count": round(insts * 64 / counts)}
 
 
-PROBE_KINDS = ["v_xor_b32", "v_bitop3_b32", "v_lshlrev_b64", "v_bcnt_u32_b32", "v_lshl_add_u64", "leaf_mix"]
+PROBE_KINDS = ["v_xor_b32", "v_bitop3_b32", "v_lshlrev_b64", "v_bcnt_u32_b32", "v_lshl_add_u64", "leaf_mix",
+               "v_lshlrev_b32", "v_alignbit_b32", "v_cndmask_b32", "v_add_u32", "v_and_or_b32", "v_sub_co_u32",
+               "v_and_b32", "v_or_b32", "v_mov_b32", "v_bfrev_b32", "v_or3_b32", "v_ffbl_b32", "v_add3_u32",
+               "v_cmp_eq_u32", "v_xor_b32_e64"]
 
 
 def valu_calibration(pg, iters: int = 100_000) -> dict:

@@ -225,7 +225,9 @@ typedef struct pg_kernel_stat {
  * 32 VALU instructions per iteration in 8 independent chains, on wavesPerSimd
  * (1-8) waves per SIMD of every CU, `iters` iterations.  kind 0 v_xor_b32,
  * 1 v_bitop3_b32, 2 v_lshlrev_b64, 3 v_bcnt_u32_b32, 4 v_lshl_add_u64, 5 the
- * move counter's mix (bitwise, 64-bit shift, select, add, popcount).  *ms is
+ * move counter's mix (bitwise, 64-bit shift, select, add, popcount),
+ * 6 v_lshlrev_b32, 7 v_alignbit_b32, 8 v_cndmask_b32, 9 v_add_u32,
+ * 10 v_and_or_b32, 11 v_sub_co_u32.  *ms is
  * the launch's time, *waveInsts the VALU instructions it issued: their
  * quotient is the VALU issue rate the leaf kernels are measured against. */
 int pg_valu_probe(int kind, int wavesPerSimd, uint64_t iters, double *ms, uint64_t *waveInsts);

@@ -13,6 +13,11 @@
 //   4 v_lshl_add_u64   5 the leaf's mix: per 8 instructions four 32-bit
 //     bitwise forms (v_bitop3_b32, v_and_or_b32, v_and_b32, v_or_b32), one
 //     v_lshlrev_b64, one v_cndmask_b32, one v_add_u32 and one v_bcnt_u32_b32
+//   6 v_lshlrev_b32    7 v_alignbit_b32   8 v_cndmask_b32    9 v_add_u32
+//   10 v_and_or_b32    11 v_sub_co_u32 (carry out to vcc)      12 v_and_b32
+//   13 v_or_b32        14 v_mov_b32       15 v_bfrev_b32       16 v_or3_b32
+//   17 v_ffbl_b32      18 v_add3_u32      19 v_cmp_eq_u32 (to vcc)
+//   20 v_xor_b32 in its 64-bit (VOP3) encoding
 // (the leaf's static code is ~48 % 32-bit bitwise, ~9 % 64-bit shifts, ~5 %
 // selects, ~5 % adds and ~3 % popcounts, the rest moves and compares).
 #include "pg_internal.h"
@@ -58,6 +63,66 @@ __global__ __launch_bounds__(PROBE_BS) void k_valu_probe(u64 iters)
 #define OP(x) asm volatile("v_lshl_add_u64 %0, %0, 1, %1" : "+v"(x) : "v"(m));
                 OP(s0) OP(s1) OP(s2) OP(s3) OP(s4) OP(s5) OP(s6) OP(s7)
 #undef OP
+            } else if constexpr (KIND == 6) {
+#define OP(x) asm volatile("v_lshlrev_b32 %0, 3, %0" : "+v"(x));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 7) {
+#define OP(x) asm volatile("v_alignbit_b32 %0, %0, %1, 24" : "+v"(x) : "v"(c));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 8) {
+#define OP(x) asm volatile("v_cndmask_b32 %0, %0, %1, %2" : "+v"(x) : "v"(c), "s"(m));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 9) {
+#define OP(x) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x) : "v"(b));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 10) {
+#define OP(x) asm volatile("v_and_or_b32 %0, %0, %1, %2" : "+v"(x) : "v"(b), "v"(c));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 11) {
+#define OP(x) asm volatile("v_sub_co_u32 %0, vcc, %0, %1" : "+v"(x) : "v"(b) : "vcc");
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 12) {
+#define OP(x) asm volatile("v_and_b32 %0, %0, %1" : "+v"(x) : "v"(c));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 13) {
+#define OP(x) asm volatile("v_or_b32 %0, %0, %1" : "+v"(x) : "v"(b));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 14) {
+#define OP(x) asm volatile("v_mov_b32 %0, %1" : "=v"(x) : "v"(x));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 15) {
+#define OP(x) asm volatile("v_bfrev_b32 %0, %0" : "+v"(x));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 16) {
+#define OP(x) asm volatile("v_or3_b32 %0, %0, %1, %2" : "+v"(x) : "v"(b), "v"(c));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 17) {
+#define OP(x) asm volatile("v_ffbl_b32 %0, %0" : "+v"(x));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 18) {
+#define OP(x) asm volatile("v_add3_u32 %0, %0, %1, %2" : "+v"(x) : "v"(b), "v"(c));
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 19) {
+#define OP(x) asm volatile("v_cmp_eq_u32 vcc, %0, %1" : : "v"(x), "v"(b) : "vcc");
+                P8(OP)
+#undef OP
+            } else if constexpr (KIND == 20) {
+#define OP(x) asm volatile("v_xor_b32_e64 %0, %0, %1" : "+v"(x) : "v"(b));
+                P8(OP)
+#undef OP
             } else {
                 asm volatile("v_bitop3_b32 %0, %0, %1, %2 bitop3:0x96" : "+v"(a0) : "v"(b), "v"(c));
                 asm volatile("v_and_or_b32 %0, %0, %1, %2" : "+v"(a1) : "v"(b), "v"(c));
@@ -75,6 +140,14 @@ __global__ __launch_bounds__(PROBE_BS) void k_valu_probe(u64 iters)
 }
 #undef P8
 
+constexpr int PROBE_KINDS = 21;
+template <int K>
+static void launchProbe(int kind, unsigned blocks, hipStream_t st, u64 iters)
+{
+    if (kind == K) hipLaunchKernelGGL(k_valu_probe<K>, dim3(blocks), dim3(PROBE_BS), 0, st, iters);
+    if constexpr (K + 1 < PROBE_KINDS) launchProbe<K + 1>(kind, blocks, st, iters);
+}
+
 }  // namespace pg
 
 using namespace pg;
@@ -87,7 +160,7 @@ extern "C" int pg_valu_probe(int kind, int wavesPerSimd, uint64_t iters, double 
 {
     Lib &L = lib();
     if (!L.init) return PG_ERR_NOT_INIT;
-    if (kind < 0 || kind > 5 || wavesPerSimd < 1 || wavesPerSimd > 8 || !iters || !ms || !waveInsts) {
+    if (kind < 0 || kind >= PROBE_KINDS || wavesPerSimd < 1 || wavesPerSimd > 8 || !iters || !ms || !waveInsts) {
         setError("pg_valu_probe: bad argument");
         return PG_ERR_INVALID;
     }
@@ -96,14 +169,7 @@ extern "C" int pg_valu_probe(int kind, int wavesPerSimd, uint64_t iters, double 
     PG_HIP(hipEventCreate(&a));
     PG_HIP(hipEventCreate(&b));
     PG_HIP(hipEventRecord(a, L.stream));
-    switch (kind) {
-    case 0: hipLaunchKernelGGL(k_valu_probe<0>, dim3(blocks), dim3(PROBE_BS), 0, L.stream, iters); break;
-    case 1: hipLaunchKernelGGL(k_valu_probe<1>, dim3(blocks), dim3(PROBE_BS), 0, L.stream, iters); break;
-    case 2: hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(PROBE_BS), 0, L.stream, iters); break;
-    case 3: hipLaunchKernelGGL(k_valu_probe<3>, dim3(blocks), dim3(PROBE_BS), 0, L.stream, iters); break;
-    case 4: hipLaunchKernelGGL(k_valu_probe<4>, dim3(blocks), dim3(PROBE_BS), 0, L.stream, iters); break;
-    default: hipLaunchKernelGGL(k_valu_probe<5>, dim3(blocks), dim3(PROBE_BS), 0, L.stream, iters); break;
-    }
+    launchProbe<0>(kind, blocks, L.stream, iters);
     PG_HIP(hipGetLastError());
     PG_HIP(hipEventRecord(b, L.stream));
     PG_HIP(hipEventSynchronize(b));
