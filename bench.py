@@ -469,6 +469,9 @@ def main():
                     help="shared transposition table of the serving leg, inside its 200 GB budget")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
     ap.add_argument("--valu-probe", action="store_true", help="only run the VALU calibration kernel and exit")
+    ap.add_argument("--tt-frontier", type=int, default=-1,
+                    help="launcher with the tables: 0 serial recursion, 1 parallel frontier, 2 batched frontier "
+                         "(-1: the library default)")
     ap.add_argument("--slider", type=int, default=0,
                     help="slider attacks of the fused leaf: 0 fills (default), 1 magic tables, 2 magic with LDS bishops")
     a = ap.parse_args()
@@ -495,6 +498,8 @@ def main():
     if a.leaf:
         pg.set_leaf_plies(a.leaf)
     pg.lib().pg_set_leaf2_variant(a.leaf2_variant)
+    if a.tt_frontier >= 0 and pg.lib().pg_set_tt_frontier(a.tt_frontier) != 0:
+        raise SystemExit(f"pg_set_tt_frontier({a.tt_frontier}): {pg.last_error()}")
     if a.slider and pg.lib().pg_set_slider_attacks(a.slider) != 0:
         raise SystemExit(f"pg_set_slider_attacks({a.slider}): {pg.last_error()}")
     pmc_known = a.leaf2_variant == 0 and a.slider == 0  # the PMC figures are of the default leaf

@@ -71,6 +71,14 @@ void pg_set_arena_limit(uint64_t bytes);
  * serial: each GPU call merges identical positions per level and the table
  * carries subtrees between calls. */
 void pg_set_frontier_batch(int on);
+/* the launcher above the launch depth WITH the tables: 0 = the reference's
+ * serial host recursion, one GPU call per node (launcher.cu:572-715);
+ * 1 = the unique frontier at the launch depth (host tier probed first) over
+ * four host threads, each with its own stream and quarter of the arena,
+ * sharing both table tiers, so the GPU calls overlap on the device; 2 = the
+ * unique frontier through the batched engine with the device table.  The
+ * frontier is expanded on the host when it lies at most 4 plies down. */
+int pg_set_tt_frontier(int mode);
 
 /* ========================= host helpers (util.cpp) ========================== */
 

@@ -84,6 +84,7 @@ _SIGS = [
     ("pg_arena_bytes", c_uint64, []),
     ("pg_set_arena_limit", None, [c_uint64]),
     ("pg_set_frontier_batch", None, [c_int]),
+    ("pg_set_tt_frontier", c_int, [c_int]),
     ("pg_parse_fen", c_int, [c_char_p, POINTER(Position)]),
     ("pg_write_fen", c_int, [POINTER(Position), c_char_p, c_int]),
     ("pg_gen_moves", c_int, [POINTER(Position), POINTER(c_uint16), POINTER(c_int)]),

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -67,6 +68,11 @@ static size_t g_peakSinceReset = 0;  // largest g_lastPeak since resetPeakSinceR
 static uint64_t g_arenaLimit = 0;  // 0 = whole arena; a test hook to force OOM
 int g_leafPlies = 2;  // fused two-ply leaf (measured 2x the one-ply leaf on config 2)
 static bool g_frontierBatch = true;  // launcher: batch the frontier above the launch depth
+// launcher with the tables: the reference's serial recursion, the unique
+// frontier over parallel host threads, or the unique frontier through the
+// batched engine with the device table (pg_set_tt_frontier)
+enum { TT_FRONTIER_SERIAL = 0, TT_FRONTIER_PARALLEL = 1, TT_FRONTIER_BATCH = 2 };
+static int g_ttFrontier = TT_FRONTIER_SERIAL;
 static const int FRONTIER_MAX_PLIES = 4;
 
 static inline QuadBitBoard toQBB(const pg_position *p)
@@ -326,11 +332,13 @@ int gpuBfs(const pg_position *pos, int depth, bool useTT, void *buf, uint64_t by
 // launch depth and makes one GPU call per node, launcher.cu:572-715): expand
 // `depth - ld` plies on the host, merge identical positions with their
 // multiplicities (exact: the whole position is the key), and run every unique
-// frontier position through the batched engine in one pass.  Without the
-// tables only: with them, each serial GPU call merges the identical positions
-// of every BFS level and stores whole subtrees for the next call, which on
-// transposition-rich trees is far less work (startpos perft 10: 1.9 s serial,
-// 48 s through the batched engine, whose levels are not merged).
+// frontier position through the batched engine in one pass.  This is the
+// default without the tables.  With them the serial recursion stays the
+// default: each GPU call finds the subtrees the calls before it stored, and
+// that beat both frontier forms on the startpos TT runs (launch depth 9;
+// perft 10 / 11: serial 0.755 / 9.22 s, the frontier over four parallel slots
+// 0.843 / 9.56 s, the frontier through the batched engine with the table
+// 0.897 / 11.91 s).
 struct PosKey {
     u64 bb[4];
     u8 state, color;
@@ -349,7 +357,9 @@ struct PosKeyHash {
     }
 };
 
-static int frontierPerft(const pg_position *pos, int depth, int ld, u128 *out)
+// The unique positions `plies` plies below `pos` and how many move sequences
+// reach each (exact: the whole position is the key).
+static void expandFrontier(const pg_position *pos, int plies, std::vector<pg_position> &ps, std::vector<u64> &mult)
 {
     std::unordered_map<PosKey, u64, PosKeyHash> cur, next;
     PosKey k0;
@@ -357,7 +367,7 @@ static int frontierPerft(const pg_position *pos, int depth, int ld, u128 *out)
     k0.state = pos->state;
     k0.color = pos->color;
     cur[k0] = 1;
-    for (int ply = 0; ply < depth - ld; ply++) {
+    for (int ply = 0; ply < plies; ply++) {
         next.clear();
         next.reserve(cur.size() * 32);
         for (auto &kv : cur) {
@@ -377,8 +387,8 @@ static int frontierPerft(const pg_position *pos, int depth, int ld, u128 *out)
         }
         std::swap(cur, next);
     }
-    std::vector<pg_position> ps;
-    std::vector<u64> mult;
+    ps.clear();
+    mult.clear();
     ps.reserve(cur.size());
     mult.reserve(cur.size());
     for (auto &kv : cur) {
@@ -389,19 +399,116 @@ static int frontierPerft(const pg_position *pos, int depth, int ld, u128 *out)
         ps.push_back(p);
         mult.push_back(kv.second);
     }
+}
+
+static void sumFrontier(const std::vector<u64> &mult, const std::vector<u64> &counts, u128 *out)
+{
+    unsigned __int128 total = 0;
+    for (size_t i = 0; i < mult.size(); i++) total += (unsigned __int128)mult[i] * counts[i];
+    out->lo = (u64)total;
+    out->hi = (u64)(total >> 64);
+}
+
+// the unique frontier through the batched engine in one pass (with the
+// device table when useTT)
+static int frontierPerft(const pg_position *pos, int depth, int ld, bool useTT, u128 *out)
+{
+    std::vector<pg_position> ps;
+    std::vector<u64> mult;
+    expandFrontier(pos, depth - ld, ps, mult);
     std::vector<u8> ds(ps.size(), (u8)ld);
     std::vector<u64> counts(ps.size());
     pg_batch_options opt{};
     opt.useGraph = 1;
+    opt.useTT = useTT ? 1 : 0;
     const size_t CH = 1 << 20;
     for (size_t b = 0; b < ps.size(); b += CH) {
         int n = (int)std::min(CH, ps.size() - b);
         PG_TRY(perftBatch(n, ps.data() + b, ds.data() + b, &opt, counts.data() + b));
     }
-    unsigned __int128 total = 0;
-    for (size_t i = 0; i < ps.size(); i++) total += (unsigned __int128)mult[i] * counts[i];
-    out->lo = (u64)total;
-    out->hi = (u64)(total >> 64);
+    sumFrontier(mult, counts, out);
+    return PG_OK;
+}
+
+// With the tables: the unique frontier at the launch depth (host tier probed
+// first) spread over TT_SLOTS host threads, each with its own HIP stream,
+// pinned readback buffer and 1/TT_SLOTS of the arena, all sharing the device
+// table and the host tier, so the GPU calls of several frontier nodes overlap
+// on the device.  A node that does not fit its slot is recomputed afterwards
+// through the serial recursion on the whole arena (which lowers the launch
+// depth as the reference does).
+static const int TT_SLOTS = 4;
+struct FrontierSlot {
+    hipStream_t stream = nullptr;
+    u64 *pinned = nullptr;
+};
+static FrontierSlot g_ttSlots[TT_SLOTS];
+
+static int recurse(const pg_position *pos, int depth, bool useTT, u128 *out);
+
+static int frontierPerftParallel(const pg_position *pos, int depth, int ld, u128 *out)
+{
+    Lib &L = lib();
+    std::vector<pg_position> ps;
+    std::vector<u64> mult;
+    expandFrontier(pos, depth - ld, ps, mult);
+    const size_t n = ps.size();
+    std::vector<u64> counts(n, 0);
+    std::vector<u8> state(n, 0);  // 0 = to compute, 1 = done, 2 = did not fit its slot
+    std::vector<Hash128> hs(n);
+    for (size_t i = 0; i < n; i++) {
+        hs[i] = hashPosition(toQBB(&ps[i]), ps[i].state, ps[i].color);
+        uint64_t v;
+        if (ld >= 2 && hostTTProbe(hs[i], ld, &v)) counts[i] = v, state[i] = 1;
+    }
+    {
+        std::lock_guard<std::mutex> g(L.mu);
+        PG_TRY(refuseWhileServing("GPU perft"));
+        for (auto &s : g_ttSlots) {
+            if (!s.stream) PG_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+            if (!s.pinned) PG_HIP(hipHostMalloc((void **)&s.pinned, 4096, hipHostMallocDefault));
+        }
+        const bool profOn = L.prof.on;
+        L.prof.on = false;  // the event profiler is single-threaded
+        const size_t per = (usableArena() / TT_SLOTS) & ~(size_t)255;
+        std::atomic<size_t> next{0};
+        std::atomic<int> err{PG_OK};
+        auto work = [&](int k) {
+            (void)hipSetDevice(L.device);
+            char *arena = (char *)L.arena + (size_t)k * per;
+            for (size_t i; (i = next.fetch_add(1)) < n && err.load() == PG_OK;) {
+                if (state[i]) continue;
+                const pg_position &p = ps[i];
+                size_t peak = 0;
+                uint64_t c = 0;
+                int s = bfsPerft(toQBB(&p), p.state, p.color, ld, g_leafPlies, true, arena, per, g_ttSlots[k].stream,
+                                 g_ttSlots[k].pinned, &c, &peak);
+                if (s == PG_ERR_ARENA_OOM) {
+                    state[i] = 2;
+                } else if (s != PG_OK) {
+                    int expect = PG_OK;
+                    err.compare_exchange_strong(expect, s);
+                } else {
+                    counts[i] = c;
+                    state[i] = 1;
+                    if (ld >= 2) hostTTStore(hs[i], ld, c);
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (int k = 0; k < TT_SLOTS; k++) th.emplace_back(work, k);
+        for (auto &t : th) t.join();
+        L.prof.on = profOn;
+        if (err.load() != PG_OK) return err.load();
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (state[i] != 2) continue;
+        g_oomCount++;
+        u128 r;
+        PG_TRY(recurse(&ps[i], ld, true, &r));
+        counts[i] = r.lo;
+    }
+    sumFrontier(mult, counts, out);
     return PG_OK;
 }
 
@@ -435,9 +542,14 @@ static int recurse(const pg_position *pos, int depth, bool useTT, u128 *out)
             total = u128(c);
         }
     }
-    if (split && g_frontierBatch && !useTT && depth > 1 && depth - g_effectiveLD <= FRONTIER_MAX_PLIES) {
-        int ld = std::min(g_effectiveLD, depth - 1);
-        PG_TRY(frontierPerft(pos, depth, ld < 1 ? 1 : ld, &total));
+    const int fld = std::max(1, std::min(g_effectiveLD, depth - 1));
+    const bool frontier = split && depth > 1 && depth - fld <= FRONTIER_MAX_PLIES;
+    if (frontier && !useTT && g_frontierBatch) {
+        PG_TRY(frontierPerft(pos, depth, fld, false, &total));
+    } else if (frontier && useTT && g_ttFrontier == TT_FRONTIER_PARALLEL && fld >= 2) {
+        PG_TRY(frontierPerftParallel(pos, depth, fld, &total));
+    } else if (frontier && useTT && g_ttFrontier == TT_FRONTIER_BATCH) {
+        PG_TRY(frontierPerft(pos, depth, fld, true, &total));
     } else if (split) {
         if (depth <= 1) {
             // a GPU call of depth 1 cannot run out of memory; depth 0 is a leaf
@@ -580,6 +692,15 @@ int pg_shutdown(void)
 uint64_t pg_arena_bytes(void) { return lib().arenaBytes; }
 void pg_set_arena_limit(uint64_t bytes) { g_arenaLimit = bytes; }
 void pg_set_frontier_batch(int on) { g_frontierBatch = on != 0; }
+int pg_set_tt_frontier(int mode)
+{
+    if (mode < TT_FRONTIER_SERIAL || mode > TT_FRONTIER_BATCH) {
+        setError("pg_set_tt_frontier: mode %d is not 0, 1 or 2", mode);
+        return PG_ERR_INVALID;
+    }
+    g_ttFrontier = mode;
+    return PG_OK;
+}
 
 int pg_parse_fen(const char *fen, pg_position *pos) { return parseFen(fen, pos); }
 int pg_write_fen(const pg_position *pos, char *out, int outLen)

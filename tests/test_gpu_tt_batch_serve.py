@@ -94,6 +94,36 @@ def test_high_mobility_leaf_children_with_tables_and_batch(gpu):
         gpu.tt_free()
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_tt_launcher_frontier_modes(gpu, mode):
+    # the launcher with the tables: serial recursion (0), the unique frontier
+    # over parallel host threads with their own streams and arena quarters
+    # (1), and the unique frontier through the batched engine (2)
+    assert gpu.lib().pg_set_tt_frontier(mode) == 0
+    gpu.tt_init(4096, 512)
+    try:
+        r = load_jsonl("c2.jsonl")[0]
+        p = gpu.parse_fen(r["fen"])
+        for ld in (4, 5):
+            print(f"[gpu] tt frontier mode={mode} c2 ld={ld}", flush=True)
+            assert gpu.perft(p, 7, launch_depth=ld, use_tt=True) == r["counts"][6]
+        rows = load_jsonl("c4.jsonl")[:8]
+        ps = [gpu.parse_fen(x["fen"]) for x in rows]
+        got = [gpu.perft(q, 6, launch_depth=3, use_tt=True) for q in ps]
+        assert got == [gpu.perft_bfs(q, 6) for q in ps]
+        # a frontier node that does not fit its slot's quarter of an 8 MB arena
+        # is recomputed through the serial recursion
+        gpu.lib().pg_tt_clear()
+        gpu.lib().pg_set_arena_limit(8 << 20)
+        try:
+            assert gpu.perft(gpu.parse_fen(KIWI), 6, launch_depth=5, use_tt=True) == 8031647685
+        finally:
+            gpu.lib().pg_set_arena_limit(0)
+    finally:
+        gpu.tt_free()
+        gpu.lib().pg_set_tt_frontier(0)
+
+
 def test_tt_tiny_table_still_exact(gpu):
     gpu.tt_init(1, 0)  # 64 Ki slots: constant overwrites and collisions of slots
     try:
