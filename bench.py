@@ -256,20 +256,32 @@ def batch_leg(pg) -> dict:
     ds = [r["depth"] for r in rows]
     want = [r["count"] for r in rows]
     total = sum(want)
-    got = pg.perft_batch(ps, ds)  # warm: autotune + graph capture
+    t0 = time.perf_counter()
+    got = pg.perft_batch(ps, ds)  # cold: autotune + graph capture + first launches
+    t_cold = time.perf_counter() - t0
     assert got == want
     t0 = time.perf_counter()
     got = pg.perft_batch(ps, ds)
     t_b = time.perf_counter() - t0
     assert got == want
+    tuned = pg.batch_tuned()
+    # the same batch with the grid held at 4, 8 and 16 blocks per CU (leaf
+    # plies and table use as tuned), warm, for the tuner's grid choice
+    sweep = {}
+    for gpc in (4, 8, 16):
+        assert pg.perft_batch(ps, ds, grid_per_cu=gpc) == want
+        t0 = time.perf_counter()
+        assert pg.perft_batch(ps, ds, grid_per_cu=gpc) == want
+        sweep[gpc] = time.perf_counter() - t0
     pg.perft_batch_plain(ps[:64], ds[:64])
     t0 = time.perf_counter()
     got = pg.perft_batch_plain(ps, ds)
     t_plain = time.perf_counter() - t0
     assert got == want
     return {"workload": "config-3: 4096 positions, 6-40 random plies, depth 3-6, no TT", "nodes": total,
-            "batched_s": t_b, "batched_nodes_per_s": total / t_b, "plain_s": t_plain,
-            "plain_nodes_per_s": total / t_plain, "batched_over_plain": t_plain / t_b}
+            "batched_s": t_b, "batched_nodes_per_s": total / t_b, "batched_cold_s": t_cold, "plain_s": t_plain,
+            "plain_nodes_per_s": total / t_plain, "batched_over_plain": t_plain / t_b,
+            "batched_cold_over_plain": t_plain / t_cold, "tuned": tuned, "grid_sweep_s": sweep}
 
 
 def tt_leg(pg) -> dict:

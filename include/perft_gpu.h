@@ -256,7 +256,9 @@ int pg_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
  * as one multi-root BFS whose leaf phase is a persistent work-queue kernel;
  * pg_perft_batch_plain calls pg_perft_bfs once per position (the baseline). */
 typedef struct pg_batch_options {
-    int blockSize;        /* 0 = autotuned */
+    int gridPerCU;        /* blocks per CU of the persistent level kernels and the
+                             table leaf (1-32), 0 = autotuned (4, 8 or 16); the
+                             blocks are 256 threads, the scan's tile size */
     int leafPlies;        /* 1 or 2, 0 = autotuned */
     int useGraph;         /* capture the level sequence in a hipGraph */
     int useTT;
@@ -265,6 +267,9 @@ typedef struct pg_batch_options {
 int pg_perft_batch(int n, const pg_position *positions, const uint8_t *depths, const pg_batch_options *opt,
                    uint64_t *counts);
 int pg_perft_batch_plain(int n, const pg_position *positions, const uint8_t *depths, int useTT, uint64_t *counts);
+/* the autotuner's cached choices, six ints per row: depth, log2 of the group
+ * size, table requested, leaf plies, table used, grid blocks per CU */
+int pg_batch_tuned(int *rows, int maxRows, int *nOut);
 
 /* ================================ serving =================================== */
 /* No reference counterpart.  A request loop that admits perft queries as they

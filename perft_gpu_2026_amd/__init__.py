@@ -49,7 +49,7 @@ class U128(ctypes.Structure):
 
 
 class BatchOptions(ctypes.Structure):
-    _fields_ = [("blockSize", c_int), ("leafPlies", c_int), ("useGraph", c_int), ("useTT", c_int),
+    _fields_ = [("gridPerCU", c_int), ("leafPlies", c_int), ("useGraph", c_int), ("useTT", c_int),
                 ("reserved", c_int * 4)]
 
 
@@ -122,6 +122,7 @@ _SIGS = [
     ("pg_leaf", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uint64, c_int, c_void_p, c_void_p]),
     ("pg_perft_batch", c_int, [c_int, POINTER(Position), POINTER(c_uint8), POINTER(BatchOptions), POINTER(c_uint64)]),
     ("pg_perft_batch_plain", c_int, [c_int, POINTER(Position), POINTER(c_uint8), c_int, POINTER(c_uint64)]),
+    ("pg_batch_tuned", c_int, [POINTER(c_int), c_int, POINTER(c_int)]),
     ("pg_serve_create", c_int, [POINTER(ServeConfig), POINTER(c_void_p)]),
     ("pg_serve_submit", c_int, [c_void_p, c_uint64, POINTER(Position), c_int]),
     ("pg_serve_poll", c_int, [c_void_p, POINTER(ServeResult), c_int, c_int, POINTER(c_int)]),
@@ -283,13 +284,22 @@ def _pack(positions, depths):
     return n, arr, d
 
 
-def perft_batch(positions, depths, leaf_plies: int = 0, block_size: int = 0, use_graph: bool = True,
+def perft_batch(positions, depths, leaf_plies: int = 0, grid_per_cu: int = 0, use_graph: bool = True,
                 use_tt: bool = False) -> list[int]:
     n, arr, d = _pack(positions, depths)
     out = (c_uint64 * max(n, 1))()
-    opt = BatchOptions(block_size, leaf_plies, int(use_graph), int(use_tt))
+    opt = BatchOptions(grid_per_cu, leaf_plies, int(use_graph), int(use_tt))
     _check(lib().pg_perft_batch(n, arr, d, byref(opt), out), "pg_perft_batch")
     return list(out[:n])
+
+
+def batch_tuned() -> list[dict]:
+    """The batch autotuner's cached choices."""
+    rows = (c_int * (6 * 256))()
+    n = c_int()
+    _check(lib().pg_batch_tuned(rows, 256, ctypes.byref(n)), "pg_batch_tuned")
+    keys = ("depth", "size_log2", "tt_requested", "leaf_plies", "tt_used", "grid_per_cu")
+    return [dict(zip(keys, rows[6 * i:6 * i + 6])) for i in range(n.value)]
 
 
 def perft_batch_plain(positions, depths, use_tt: bool = False) -> list[int]:

@@ -678,20 +678,30 @@ __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
 struct GroupPlan {
     int depth, plies, leafLevel;
     bool useTT;
+    int gridPerCU = 8;  // blocks per CU of the persistent level kernels and the TT leaf
+};
+
+// a tuned choice: leaf plies, table use, persistent grid
+struct TunedPlan {
+    int plies;
+    bool useTT;
+    int gridPerCU;
 };
 
 struct BatchState {
     BatchDev *dev = nullptr;  // device descriptor
     BatchDev host{};          // host copy used to initialise it
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (depth, plies, tt) -> graph
-    std::map<std::tuple<int, int, int>, std::pair<int, bool>> tuned;  // (depth, size bucket, tt) -> plies, tt use
+    // (depth, plies, tt, grid per CU) -> graph
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+    // (depth, size bucket, tt requested) -> choice
+    std::map<std::tuple<int, int, int>, TunedPlan> tuned;
 };
 static BatchState g_batch;
 
-static unsigned persistentBlocks()
+static unsigned persistentBlocks(int perCU)
 {
     int cu = lib().cuCount ? lib().cuCount : 256;
-    return (unsigned)(cu * 8);
+    return (unsigned)(cu * perCU);
 }
 
 // enqueue the whole level sequence of one group (capturable: fixed shapes,
@@ -699,7 +709,7 @@ static unsigned persistentBlocks()
 static int enqueueGroup(const GroupPlan &g, hipStream_t st)
 {
     BatchDev *d = g_batch.dev;
-    unsigned G = persistentBlocks();
+    unsigned G = persistentBlocks(g.gridPerCU);
     hipLaunchKernelGGL(k_b_init, dim3(64), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_b_count0, dim3(G), dim3(BS), 0, st, d);
     for (int L = 0; L < g.leafLevel; L++) {
@@ -806,7 +816,7 @@ static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *sta
     Profiler &P = L.prof;
     P.begin(K_BATCH_GROUP, st);
     if (useGraph) {
-        auto key = std::make_tuple(g.depth, g.plies, (int)g.useTT);
+        auto key = std::make_tuple(g.depth, g.plies, (int)g.useTT, g.gridPerCU);
         auto it = g_batch.graphs.find(key);
         if (it == g_batch.graphs.end()) {
             hipGraph_t graph;
@@ -855,7 +865,7 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
             flipColors(kids[i], ks[i]);
         }
         int plies = g.plies < g.depth - 1 ? g.plies : g.depth - 2;
-        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies, g.useTT};
+        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies, g.useTT, g.gridPerCU};
         std::vector<u64> kc(nc);
         if (nc) PG_TRY(runGroupSplit(sub, kids.data(), ks.data(), nc, useGraph, kc.data()));
         u64 total = 0;
@@ -869,17 +879,19 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
 }
 
 // Autotuner: picks a group's leaf plies (1 or 2: where the BFS hands over to
-// the fused leaf) and, when the table is on, whether the group uses it (its
-// probes and stores cost more than they save on shallow groups), by timing
-// every combination on a sample of the group.  Each combination first runs
-// on one sample (graph capture, first launches, table warm-up) and is timed
-// on a second, disjoint one when the group has room, so a timed run does not
-// hit its own entries.  The sample shrinks 4x per ply above depth 4 so that
-// tuning a deep group stays cheap.  Choices are cached per (depth, log2 size,
-// table requested).
+// the fused leaf), when the table is on whether the group uses it (its probes
+// and stores cost more than they save on shallow groups), and then the
+// persistent grid of its level kernels and TT leaf (4, 8 or 16 blocks per
+// CU), by timing each choice on a sample of the group.  Each choice first
+// runs on one sample (graph capture, first launches, table warm-up) and is
+// timed on a second, disjoint one when the group has room, so a timed run
+// does not hit its own entries.  The sample shrinks 4x per ply above depth 4
+// so that tuning a deep group stays cheap.  Choices are cached per (depth,
+// log2 size, table requested).
 constexpr int TUNE_MAX_DEPTH = 7;  // groups this deep are not tuned
-static std::pair<int, bool> chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roots,
-                                            const std::vector<u8> &states, bool useGraph, bool useTT)
+static const int TUNE_GRIDS[] = {4, 8, 16};
+static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roots, const std::vector<u8> &states,
+                                 bool useGraph, bool useTT, int forcedGrid)
 {
     int bucket = 0;
     for (u64 n = roots.size(); n > 1; n >>= 1) bucket++;
@@ -889,24 +901,35 @@ static std::pair<int, bool> chooseGroupPlan(int depth, const std::vector<QuadBit
     // Deep groups are leaf-bound, where the fused two-ply leaf always wins
     // and the table always pays; a one-ply trial there would cost more than
     // a whole shallow batch (startpos children at depth 9: minutes).
-    if (depth >= TUNE_MAX_DEPTH) return g_batch.tuned[key] = {2, useTT};
+    if (depth >= TUNE_MAX_DEPTH) return g_batch.tuned[key] = {2, useTT, forcedGrid ? forcedGrid : 8};
     u64 ns = 256;
     for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
     ns = std::min<u64>(roots.size(), ns);
     const u64 t0i = roots.size() >= 2 * ns ? ns : 0;  // timed sample [t0i, t0i + ns)
     std::vector<u64> tmp(ns);
+    auto trial = [&](const GroupPlan &g) {
+        double t = 1e30;
+        if (runGroupSplit(g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
+            auto a = std::chrono::steady_clock::now();
+            if (runGroupSplit(g, roots.data() + t0i, states.data() + t0i, ns, useGraph, tmp.data()) == PG_OK)
+                t = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        }
+        return t;
+    };
+    const int grid0 = forcedGrid ? forcedGrid : 8;
     double best = 1e30;
-    std::pair<int, bool> choice{2, useTT};
+    TunedPlan choice{2, useTT, grid0};
     for (int tt = useTT ? 1 : 0; tt >= 0; tt--) {
         for (int p = 1; p <= 2 && p < depth; p++) {
-            GroupPlan g{depth, p, depth - p, tt != 0};
-            double t = 1e30;
-            if (runGroupSplit(g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
-                auto a = std::chrono::steady_clock::now();
-                if (runGroupSplit(g, roots.data() + t0i, states.data() + t0i, ns, useGraph, tmp.data()) == PG_OK)
-                    t = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-            }
-            if (t < best) best = t, choice = {p, tt != 0};
+            double t = trial(GroupPlan{depth, p, depth - p, tt != 0, grid0});
+            if (t < best) best = t, choice = {p, tt != 0, grid0};
+        }
+    }
+    if (!forcedGrid) {
+        for (int gpc : TUNE_GRIDS) {
+            if (gpc == grid0) continue;
+            double t = trial(GroupPlan{depth, choice.plies, depth - choice.plies, choice.useTT, gpc});
+            if (t < best) best = t, choice.gridPerCU = gpc;
         }
     }
     g_batch.tuned[key] = choice;
@@ -926,6 +949,7 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
     bool useGraph = !opt || opt->useGraph;
     bool useTT = opt && opt->useTT && L.tt.entries;
     int forcedPlies = opt ? opt->leafPlies : 0;
+    const int forcedGrid = opt && opt->gridPerCU > 0 ? (opt->gridPerCU > 32 ? 32 : opt->gridPerCU) : 0;
     // group by depth; depth <= 1 on the host as the reference does (perft_kernels.cu:623-626)
     std::map<int, std::vector<int>> groups;
     for (int i = 0; i < n; i++) {
@@ -954,13 +978,15 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
         }
         int plies = forcedPlies;
         bool groupTT = useTT;
+        int grid = forcedGrid ? forcedGrid : 8;
         if (plies != 1 && plies != 2) {
-            std::pair<int, bool> c = chooseGroupPlan(depth, roots, states, useGraph, useTT);
-            plies = c.first;
-            groupTT = c.second;
+            TunedPlan c = chooseGroupPlan(depth, roots, states, useGraph, useTT, forcedGrid);
+            plies = c.plies;
+            groupTT = c.useTT;
+            grid = forcedGrid ? forcedGrid : c.gridPerCU;
         }
         if (plies >= depth) plies = depth - 1;
-        GroupPlan g{depth, plies, depth - plies, groupTT};
+        GroupPlan g{depth, plies, depth - plies, groupTT, grid};
         std::vector<u64> out(ids.size());
         PG_TRY(runGroupSplit(g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
         for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];
@@ -976,6 +1002,25 @@ extern "C" {
 void pg_set_leaf_plies(int plies)
 {
     if (plies == 1 || plies == 2) g_leafPlies = plies;
+}
+
+int pg_batch_tuned(int *rows, int maxRows, int *nOut)
+{
+    if (!nOut || (maxRows > 0 && !rows)) return PG_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(lib().mu);
+    int n = 0;
+    for (auto &kv : g_batch.tuned) {
+        if (n >= maxRows) break;
+        int *r = rows + 6 * n++;
+        r[0] = std::get<0>(kv.first);
+        r[1] = std::get<1>(kv.first);
+        r[2] = std::get<2>(kv.first);
+        r[3] = kv.second.plies;
+        r[4] = kv.second.useTT ? 1 : 0;
+        r[5] = kv.second.gridPerCU;
+    }
+    *nOut = n;
+    return PG_OK;
 }
 
 int pg_perft_batch(int n, const pg_position *positions, const uint8_t *depths, const pg_batch_options *opt,

@@ -194,6 +194,27 @@ def test_batch_deep_group_not_tuned(gpu):
     assert time.perf_counter() - t0 < 30.0
 
 
+def test_batch_grid_choices(gpu):
+    # the persistent grid of the level kernels and the table leaf held at
+    # 4, 16 and 32 blocks per CU, and the tuner's own choice, give the
+    # golden counts; the tuner reports a grid from its candidate set
+    rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] <= 5][:400]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    ds = [r["depth"] for r in rows]
+    want = [r["count"] for r in rows]
+    for gpc in (4, 16, 32):
+        assert gpu.perft_batch(ps, ds, grid_per_cu=gpc) == want, gpc
+    gpu.tt_init(512, 0)
+    try:
+        assert gpu.perft_batch(ps, ds, grid_per_cu=4, use_tt=True) == want
+        assert gpu.perft_batch(ps, ds, use_tt=True) == want
+    finally:
+        gpu.tt_free()
+    assert gpu.perft_batch(ps, ds) == want
+    tuned = gpu.batch_tuned()
+    assert tuned and all(t["grid_per_cu"] in (4, 8, 16) and t["leaf_plies"] in (1, 2) for t in tuned), tuned
+
+
 def test_batch_graph_replay_and_eager_agree(gpu):
     rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (3, 4)][:600]
     ps = [gpu.parse_fen(r["fen"]) for r in rows]
