@@ -363,7 +363,7 @@ def ref_positions_leg(pg, tt_max_depth: int, tt_only: bool = False) -> dict:
     return out
 
 
-def serve_leg(pg, n_req: int, rate: float, tt_gb: int, slots: int) -> dict:
+def serve_leg(pg, n_req: int, rate: float, tt_gb: int, slots: int, merge: int = 1) -> dict:
     import ctypes
     import random
     import threading
@@ -374,7 +374,7 @@ def serve_leg(pg, n_req: int, rate: float, tt_gb: int, slots: int) -> dict:
     L = pg.lib()
 
     def run(arrivals):
-        cfg = pg.ServeConfig(200 << 30, tt_gb << 30, slots, 1)
+        cfg = pg.ServeConfig(200 << 30, tt_gb << 30, slots, merge)
         h = ctypes.c_void_p()
         if L.pg_serve_create(ctypes.byref(cfg), ctypes.byref(h)) != 0:
             raise RuntimeError(pg.last_error())
@@ -435,7 +435,8 @@ def serve_leg(pg, n_req: int, rate: float, tt_gb: int, slots: int) -> dict:
     want = {**gold, **plain}
     bad = [k for k, v in done.items() if v[4] != 0 or v[0] != want[k]]
     return {"workload": f"config-5: first {len(reqs)} requests (0-60 plies, depth 3-7), TT shared "
-                        f"({tt_gb} GB), 200 GB budget, {slots} slots, Poisson arrivals",
+                        f"({tt_gb} GB), 200 GB budget, {slots} slots, up to {merge} queued requests of one "
+                        f"depth <= 5 merged per pass, Poisson arrivals",
             "rate_req_per_s": lam, "capacity_req_per_s": capacity, "p50_ms": 1e3 * lat[len(lat) // 2],
             "p99_ms": 1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], "nodes_per_s": nodes / span,
             "golden_checked": sum(1 for k in done if k in gold), "checked_without_table": len(plain),
@@ -477,6 +478,8 @@ def main():
     ap.add_argument("--serve", type=int, default=10000, help="config-5 requests to replay (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.0)
     ap.add_argument("--serve-slots", type=int, default=8, help="requests in flight in the serving leg")
+    ap.add_argument("--serve-batch", type=int, default=1,
+                    help="queued requests of one depth (at most 5) a free slot merges into one multi-root pass")
     ap.add_argument("--serve-tt-gb", type=int, default=16,
                     help="shared transposition table of the serving leg, inside its 200 GB budget")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
@@ -589,7 +592,7 @@ def main():
         out["reference_positions"] = ref_positions_leg(pg, a.ref_tt_depth, a.ref_tt_only)
     if a.serve:
         print("[bench] config-5 serving", file=sys.stderr, flush=True)
-        out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate, a.serve_tt_gb, a.serve_slots)
+        out["serve_config5"] = serve_leg(pg, a.serve, a.serve_rate, a.serve_tt_gb, a.serve_slots, a.serve_batch)
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(fx["fen"], depth)

@@ -688,6 +688,10 @@ struct TunedPlan {
     int gridPerCU;
 };
 
+// One batched engine: its descriptor and graphs, and where it runs.  The
+// library's own (g_batch) runs on the library arena and stream under the
+// library mutex; a serving slot owns one over its arena partition, stream and
+// pinned buffer (pg_serve.hip).
 struct BatchState {
     BatchDev *dev = nullptr;  // device descriptor
     BatchDev host{};          // host copy used to initialise it
@@ -695,6 +699,10 @@ struct BatchState {
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
     // (depth, size bucket, tt requested) -> choice
     std::map<std::tuple<int, int, int>, TunedPlan> tuned;
+    char *arena = nullptr;  // null: the library arena (usableArena bytes)
+    size_t arenaBytes = 0;
+    hipStream_t stream = nullptr;  // null: the library stream
+    u64 *pinned = nullptr;         // null: the library's pinned buffer
 };
 static BatchState g_batch;
 
@@ -706,9 +714,9 @@ static unsigned persistentBlocks(int perCU)
 
 // enqueue the whole level sequence of one group (capturable: fixed shapes,
 // sizes read on the device)
-static int enqueueGroup(const GroupPlan &g, hipStream_t st)
+static int enqueueGroup(BatchState &B, const GroupPlan &g, hipStream_t st)
 {
-    BatchDev *d = g_batch.dev;
+    BatchDev *d = B.dev;
     unsigned G = persistentBlocks(g.gridPerCU);
     hipLaunchKernelGGL(k_b_init, dim3(64), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_b_count0, dim3(G), dim3(BS), 0, st, d);
@@ -777,13 +785,16 @@ static void flipColors(QuadBitBoard &q, u8 &state)
 }
 
 // run one group (roots already white to move) on the arena; OOM splits it
-static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *states, u64 nRoots, bool useGraph,
-                    u64 *counts)
+static int runGroup(BatchState &B, const GroupPlan &g, const QuadBitBoard *roots, const u8 *states, u64 nRoots,
+                    bool useGraph, u64 *counts)
 {
     Lib &L = lib();
-    hipStream_t st = L.stream;
+    hipStream_t st = B.stream ? B.stream : L.stream;
+    char *arena = B.arena ? B.arena : (char *)L.arena;
+    const size_t arenaBytes = B.arena ? B.arenaBytes : usableArena();
+    u64 *pinned = B.pinned ? B.pinned : L.hostPinned;
     // arena layout: [roots | states | level-0 arrays | rootCounts | device bump region]
-    Bump A(L.arena, usableArena());
+    Bump A(arena, arenaBytes);
     QuadBitBoard *dRoots = A.alloc<QuadBitBoard>(nRoots);
     u8 *dStates = A.alloc<u8>(nRoots);
     u32 *cnt0 = A.alloc<u32>(nRoots);
@@ -795,10 +806,10 @@ static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *sta
     if (!dRoots || !dStates || !cnt0 || !ts0 || !to0 || !rc || (g.useTT && !known0)) return PG_ERR_ARENA_OOM;
     PG_HIP(hipMemcpyAsync(dRoots, roots, nRoots * sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
     PG_HIP(hipMemcpyAsync(dStates, states, nRoots, hipMemcpyHostToDevice, st));
-    BatchDev &h = g_batch.host;
+    BatchDev &h = B.host;
     memset(&h, 0, sizeof h);
-    h.arena = (char *)L.arena + A.off;
-    h.arenaCap = usableArena() - A.off;
+    h.arena = arena + A.off;
+    h.arenaCap = arenaBytes - A.off;
     h.leafLevel = (u32)g.leafLevel;
     h.rootCounts = rc;
     h.lv[0].n = nRoots;
@@ -812,43 +823,45 @@ static int runGroup(const GroupPlan &g, const QuadBitBoard *roots, const u8 *sta
     h.useTT = g.useTT ? 1 : 0;
     h.tt = (TTEntry *)L.tt.entries;
     h.ttMask = L.tt.mask;
-    PG_HIP(hipMemcpyAsync(g_batch.dev, &h, sizeof h, hipMemcpyHostToDevice, st));
+    PG_HIP(hipMemcpyAsync(B.dev, &h, sizeof h, hipMemcpyHostToDevice, st));
+    // the event profiler is single-threaded: only the library's engine records
     Profiler &P = L.prof;
-    P.begin(K_BATCH_GROUP, st);
+    const bool prof = &B == &g_batch;
+    if (prof) P.begin(K_BATCH_GROUP, st);
     if (useGraph) {
         auto key = std::make_tuple(g.depth, g.plies, (int)g.useTT, g.gridPerCU);
-        auto it = g_batch.graphs.find(key);
-        if (it == g_batch.graphs.end()) {
+        auto it = B.graphs.find(key);
+        if (it == B.graphs.end()) {
             hipGraph_t graph;
             PG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            int s = enqueueGroup(g, st);
+            int s = enqueueGroup(B, g, st);
             PG_HIP(hipStreamEndCapture(st, &graph));
             if (s != PG_OK) return s;
             hipGraphExec_t exec;
             PG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
-            it = g_batch.graphs.emplace(key, exec).first;
+            it = B.graphs.emplace(key, exec).first;
         }
         PG_HIP(hipGraphLaunch(it->second, st));
     } else {
-        PG_TRY(enqueueGroup(g, st));
+        PG_TRY(enqueueGroup(B, g, st));
     }
-    P.end(0, (double)nRoots, st);
+    if (prof) P.end(0, (double)nRoots, st);
     // one readback for the whole group: the OOM flag, then the counts
-    PG_HIP(hipMemcpyAsync(L.hostPinned, &g_batch.dev->oom, sizeof(u32), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipMemcpyAsync(pinned, &B.dev->oom, sizeof(u32), hipMemcpyDeviceToHost, st));
     PG_HIP(hipMemcpyAsync(counts, rc, nRoots * sizeof(u64), hipMemcpyDeviceToHost, st));
-    if (g.useTT) PG_HIP(hipMemcpyAsync(L.hostPinned + 1, g_batch.dev->ttCtr, 3 * sizeof(u64), hipMemcpyDeviceToHost, st));
+    if (g.useTT) PG_HIP(hipMemcpyAsync(pinned + 1, B.dev->ttCtr, 3 * sizeof(u64), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
-    P.collect();
-    if (g.useTT) ttCountDevice(L.hostPinned[1], L.hostPinned[2], L.hostPinned[3]);
-    if (*(u32 *)L.hostPinned) return PG_ERR_ARENA_OOM;
+    if (prof) P.collect();
+    if (g.useTT) ttCountDevice(pinned[1], pinned[2], pinned[3]);
+    if (*(u32 *)pinned) return PG_ERR_ARENA_OOM;
     return PG_OK;
 }
 
-static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8 *states, u64 nRoots, bool useGraph,
-                         u64 *counts)
+static int runGroupSplit(BatchState &B, const GroupPlan &g, const QuadBitBoard *roots, const u8 *states, u64 nRoots,
+                         bool useGraph, u64 *counts)
 {
-    int s = runGroup(g, roots, states, nRoots, useGraph, counts);
+    int s = runGroup(B, g, roots, states, nRoots, useGraph, counts);
     if (s != PG_ERR_ARENA_OOM) return s;
     if (nRoots == 1) {
         // a single root too large for one pass: its children become a batch
@@ -867,15 +880,15 @@ static int runGroupSplit(const GroupPlan &g, const QuadBitBoard *roots, const u8
         int plies = g.plies < g.depth - 1 ? g.plies : g.depth - 2;
         GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies, g.useTT, g.gridPerCU};
         std::vector<u64> kc(nc);
-        if (nc) PG_TRY(runGroupSplit(sub, kids.data(), ks.data(), nc, useGraph, kc.data()));
+        if (nc) PG_TRY(runGroupSplit(B, sub, kids.data(), ks.data(), nc, useGraph, kc.data()));
         u64 total = 0;
         for (u64 v : kc) total += v;
         counts[0] = total;
         return PG_OK;
     }
     u64 h = nRoots / 2;
-    PG_TRY(runGroupSplit(g, roots, states, h, useGraph, counts));
-    return runGroupSplit(g, roots + h, states + h, nRoots - h, useGraph, counts + h);
+    PG_TRY(runGroupSplit(B, g, roots, states, h, useGraph, counts));
+    return runGroupSplit(B, g, roots + h, states + h, nRoots - h, useGraph, counts + h);
 }
 
 // Autotuner: picks a group's leaf plies (1 or 2: where the BFS hands over to
@@ -909,9 +922,9 @@ static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roo
     std::vector<u64> tmp(ns);
     auto trial = [&](const GroupPlan &g) {
         double t = 1e30;
-        if (runGroupSplit(g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
+        if (runGroupSplit(g_batch, g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
             auto a = std::chrono::steady_clock::now();
-            if (runGroupSplit(g, roots.data() + t0i, states.data() + t0i, ns, useGraph, tmp.data()) == PG_OK)
+            if (runGroupSplit(g_batch, g, roots.data() + t0i, states.data() + t0i, ns, useGraph, tmp.data()) == PG_OK)
                 t = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
         }
         return t;
@@ -934,6 +947,64 @@ static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roo
     }
     g_batch.tuned[key] = choice;
     return choice;
+}
+
+// ------------------------------------------------ engines of serving slots
+BatchState *batchSlotCreate(void *arena, size_t bytes, hipStream_t st, u64 *pinned)
+{
+    BatchState *B = new BatchState();
+    if (hipMalloc((void **)&B->dev, sizeof(BatchDev)) != hipSuccess) {
+        (void)hipGetLastError();
+        delete B;
+        return nullptr;
+    }
+    B->arena = (char *)arena;
+    B->arenaBytes = bytes;
+    B->stream = st;
+    B->pinned = pinned;
+    return B;
+}
+
+void batchSlotDestroy(BatchState *B)
+{
+    if (!B) return;
+    for (auto &kv : B->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (B->dev) (void)hipFree(B->dev);
+    delete B;
+}
+
+// Every position at its depth (2..MAXLV-2) as multi-root groups on the slot's
+// engine, with a fixed plan (no trials in a serving loop): the two-ply leaf,
+// the table when useTT and one exists, 8 blocks per CU; graphs captured per
+// slot on first use.
+int batchSlotRun(BatchState *B, int n, const pg_position *pos, const uint8_t *depths, bool useTT, uint64_t *counts)
+{
+    std::map<int, std::vector<int>> groups;
+    for (int i = 0; i < n; i++) {
+        if (depths[i] < 2 || depths[i] > MAXLV - 2) {
+            setError("batchSlotRun: depth %d outside 2..%d", depths[i], MAXLV - 2);
+            return PG_ERR_INVALID;
+        }
+        groups[depths[i]].push_back(i);
+    }
+    const bool tt = useTT && lib().tt.entries;
+    for (auto &kv : groups) {
+        const int depth = kv.first;
+        const std::vector<int> &ids = kv.second;
+        std::vector<QuadBitBoard> roots(ids.size());
+        std::vector<u8> states(ids.size());
+        for (size_t k = 0; k < ids.size(); k++) {
+            memcpy(roots[k].bb, pos[ids[k]].bb, 32);
+            states[k] = pos[ids[k]].state;
+            if (pos[ids[k]].color == BLACK) flipColors(roots[k], states[k]);
+        }
+        const int plies = depth > 2 ? 2 : 1;
+        GroupPlan g{depth, plies, depth - plies, tt, 8};
+        std::vector<u64> out(ids.size());
+        PG_TRY(runGroupSplit(*B, g, roots.data(), states.data(), ids.size(), true, out.data()));
+        for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];
+    }
+    return PG_OK;
 }
 
 int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_batch_options *opt, uint64_t *counts)
@@ -988,7 +1059,7 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
         if (plies >= depth) plies = depth - 1;
         GroupPlan g{depth, plies, depth - plies, groupTT, grid};
         std::vector<u64> out(ids.size());
-        PG_TRY(runGroupSplit(g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
+        PG_TRY(runGroupSplit(g_batch, g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
         for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];
     }
     return PG_OK;

@@ -2,6 +2,7 @@
 // path, batch) that the C ABI in include/perft_gpu.h and the reference-named
 // C++ layer in include/perft_launcher.h are built on.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/perft_gpu.h"
@@ -43,5 +44,11 @@ void resetPeakSinceReset();
 size_t usableArena();
 extern int g_leafPlies;
 int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_batch_options *opt, uint64_t *counts);
+// a batched engine over a caller's arena partition, stream and pinned buffer
+// (a serving slot's), for merging queued requests into one multi-root pass
+struct BatchState;
+BatchState *batchSlotCreate(void *arena, size_t bytes, hipStream_t st, u64 *pinned);
+void batchSlotDestroy(BatchState *B);
+int batchSlotRun(BatchState *B, int n, const pg_position *pos, const uint8_t *depths, bool useTT, uint64_t *counts);
 
 }  // namespace pg

@@ -44,8 +44,12 @@ struct Slot {
     void *arena = nullptr;
     size_t bytes = 0;
     u64 *pinned = nullptr;
+    BatchState *batch = nullptr;  // the slot's batched engine (maxBatch > 1)
     std::thread th;
 };
+
+// requests deeper than this run alone (a whole-depth BFS fills the GPU)
+static const int SERVE_MERGE_MAX_DEPTH = 5;
 
 struct Server {
     pg_serve_config cfg{};
@@ -85,28 +89,62 @@ struct Server {
         return PG_OK;
     }
 
+    // Takes the request at the head of the queue and, when it is shallow and
+    // maxBatch > 1, the other waiting requests of the same depth up to
+    // maxBatch in all (they run in one multi-root pass on the slot's batched
+    // engine); every request taken keeps its own submission time.
     void worker(int k)
     {
         Slot &s = slots[k];
         (void)hipSetDevice(lib().device);
+        std::vector<ServeRequest> take;
         for (;;) {
-            ServeRequest r;
+            take.clear();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cvWork.wait(lk, [&] { return stop || !queue.empty(); });
                 if (stop && queue.empty()) return;
-                r = queue.front();
+                take.push_back(queue.front());
                 queue.pop_front();
+                const int d = take[0].depth;
+                if (s.batch && cfg.maxBatch > 1 && d >= 2 && d <= SERVE_MERGE_MAX_DEPTH) {
+                    for (auto it = queue.begin(); it != queue.end() && (int)take.size() < cfg.maxBatch;) {
+                        if (it->depth == d) {
+                            take.push_back(*it);
+                            it = queue.erase(it);
+                        } else {
+                            ++it;
+                        }
+                    }
+                }
             }
-            pg_serve_result res{};
-            res.id = r.id;
-            res.submitTime = r.submit;
-            res.startTime = nowSec();
-            res.status = runOne(s, r.pos, r.depth, &res.count);
-            res.endTime = nowSec();
+            const double start = nowSec();
+            std::vector<uint64_t> counts(take.size(), 0);
+            std::vector<int> status(take.size(), PG_OK);
+            if (take.size() > 1) {
+                std::vector<pg_position> ps(take.size());
+                std::vector<uint8_t> ds(take.size());
+                for (size_t i = 0; i < take.size(); i++) ps[i] = take[i].pos, ds[i] = (uint8_t)take[i].depth;
+                if (batchSlotRun(s.batch, (int)take.size(), ps.data(), ds.data(), true, counts.data()) != PG_OK) {
+                    // not expected (OOM splits inside); each request alone instead
+                    for (size_t i = 0; i < take.size(); i++) status[i] = runOne(s, take[i].pos, take[i].depth, &counts[i]);
+                }
+            } else {
+                status[0] = runOne(s, take[0].pos, take[0].depth, &counts[0]);
+            }
+            const double end = nowSec();
             {
                 std::lock_guard<std::mutex> lk(mu);
-                done.push_back(res);
+                for (size_t i = 0; i < take.size(); i++) {
+                    pg_serve_result res{};
+                    res.id = take[i].id;
+                    res.submitTime = take[i].submit;
+                    res.startTime = start;
+                    res.endTime = end;
+                    res.count = counts[i];
+                    res.status = status[i];
+                    done.push_back(res);
+                }
             }
             cvDone.notify_all();
         }
@@ -175,6 +213,11 @@ static int serveCreateLocked(const pg_serve_config *cfg, void **handle)
             delete S;
             return PG_ERR_HIP;
         }
+        if (cfg->maxBatch > 1 && !(s.batch = batchSlotCreate(s.arena, s.bytes, s.stream, s.pinned))) {
+            setError("serving slot %d: batched engine allocation failed", k);
+            delete S;
+            return PG_ERR_HIP;
+        }
     }
     for (int k = 0; k < slots; k++) S->slots[k].th = std::thread(&Server::worker, S, k);
     *handle = S;
@@ -196,6 +239,7 @@ static int serveDestroy(Server *S)
             (void)hipStreamDestroy(s.stream);
         }
         if (s.pinned) (void)hipHostFree(s.pinned);
+        batchSlotDestroy(s.batch);
     }
     if (S->ownsTT) {
         Lib &L = lib();

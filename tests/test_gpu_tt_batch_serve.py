@@ -251,9 +251,9 @@ def test_batch_oom_split(gpu):
     assert got == [r["count"] for r in rows]
 
 
-def _serve(gpu, reqs, budget=8 << 30, slots=4, interarrival=0.0):
+def _serve(gpu, reqs, budget=8 << 30, slots=4, interarrival=0.0, merge=1):
     import ctypes
-    cfg = gpu.ServeConfig(budget, 512 << 20, slots, 1)
+    cfg = gpu.ServeConfig(budget, 512 << 20, slots, merge)
     h = ctypes.c_void_p()
     assert gpu.lib().pg_serve_create(ctypes.byref(cfg), ctypes.byref(h)) == 0, gpu.last_error()
     out = {}
@@ -282,6 +282,9 @@ def test_serving_alone_equals_in_mix(gpu):
     want = {r["seed"]: r["count"] for r in sample}
     mixed = _serve(gpu, reqs, slots=4)
     assert mixed == want
+    # queued requests of one depth merged into multi-root passes on the slots'
+    # batched engines (all submitted at once, so the queue holds many)
+    assert _serve(gpu, reqs, slots=2, merge=16) == want
     for rid in list(reqs)[:10]:  # each alone, one slot, nothing else in flight
         assert _serve(gpu, {rid: reqs[rid]}, slots=1) == {rid: want[rid]}
 
