@@ -434,9 +434,10 @@ def serve_leg(pg, n_req: int, rate: float, tt_gb: int, slots: int, merge: int = 
     t_check = time.perf_counter() - t0
     want = {**gold, **plain}
     bad = [k for k, v in done.items() if v[4] != 0 or v[0] != want[k]]
+    merging = (f"up to {merge} queued requests of one depth <= 5 merged per pass" if merge > 1
+               else "one request per pass")
     return {"workload": f"config-5: first {len(reqs)} requests (0-60 plies, depth 3-7), TT shared "
-                        f"({tt_gb} GB), 200 GB budget, {slots} slots, up to {merge} queued requests of one "
-                        f"depth <= 5 merged per pass, Poisson arrivals",
+                        f"({tt_gb} GB), 200 GB budget, {slots} slots, {merging}, Poisson arrivals",
             "rate_req_per_s": lam, "capacity_req_per_s": capacity, "p50_ms": 1e3 * lat[len(lat) // 2],
             "p99_ms": 1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], "nodes_per_s": nodes / span,
             "golden_checked": sum(1 for k in done if k in gold), "checked_without_table": len(plain),
