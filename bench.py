@@ -278,10 +278,24 @@ def batch_leg(pg) -> dict:
     got = pg.perft_batch_plain(ps, ds)
     t_plain = time.perf_counter() - t0
     assert got == want
+    # beside the -nott workload: the same batch with a 64 GB device table,
+    # emptied before the timed pass so that it finds only what the pass stores
+    pg.tt_init(65536, 0)
+    try:
+        assert pg.perft_batch(ps, ds, use_tt=True) == want  # tunes the table groups
+        pg.lib().pg_tt_clear()
+        t0 = time.perf_counter()
+        assert pg.perft_batch(ps, ds, use_tt=True) == want
+        t_tt = time.perf_counter() - t0
+        tuned_tt = [t for t in pg.batch_tuned() if t["tt_requested"]]
+    finally:
+        pg.tt_free()
     return {"workload": "config-3: 4096 positions, 6-40 random plies, depth 3-6, no TT", "nodes": total,
             "batched_s": t_b, "batched_nodes_per_s": total / t_b, "batched_cold_s": t_cold, "plain_s": t_plain,
             "plain_nodes_per_s": total / t_plain, "batched_over_plain": t_plain / t_b,
-            "batched_cold_over_plain": t_plain / t_cold, "tuned": tuned, "grid_sweep_s": sweep}
+            "batched_cold_over_plain": t_plain / t_cold, "tuned": tuned, "grid_sweep_s": sweep,
+            "with_table": {"device_table_gb": 64, "batched_s": t_tt, "batched_nodes_per_s": total / t_tt,
+                           "tuned": tuned_tt}}
 
 
 def tt_leg(pg) -> dict:

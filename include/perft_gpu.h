@@ -262,13 +262,18 @@ typedef struct pg_batch_options {
     int leafPlies;        /* 1 or 2, 0 = autotuned */
     int useGraph;         /* capture the level sequence in a hipGraph */
     int useTT;
-    int reserved[4];
+    int ttBudgetMB;       /* with useTT: the part of the device table a group
+                             uses (its first power-of-two entries within this
+                             many MB, at least 1 MB, at most the table); 0 =
+                             autotuned (the whole table, 1 GiB or 128 MiB) */
+    int reserved[3];
 } pg_batch_options;
 int pg_perft_batch(int n, const pg_position *positions, const uint8_t *depths, const pg_batch_options *opt,
                    uint64_t *counts);
 int pg_perft_batch_plain(int n, const pg_position *positions, const uint8_t *depths, int useTT, uint64_t *counts);
-/* the autotuner's cached choices, six ints per row: depth, log2 of the group
- * size, table requested, leaf plies, table used, grid blocks per CU */
+/* the autotuner's cached choices, seven ints per row: depth, log2 of the group
+ * size, table requested, leaf plies, table used, grid blocks per CU, log2 of
+ * the device-table entries used (0 when the table is not used) */
 int pg_batch_tuned(int *rows, int maxRows, int *nOut);
 
 /* ================================ serving =================================== */

@@ -50,7 +50,7 @@ class U128(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("gridPerCU", c_int), ("leafPlies", c_int), ("useGraph", c_int), ("useTT", c_int),
-                ("reserved", c_int * 4)]
+                ("ttBudgetMB", c_int), ("reserved", c_int * 3)]
 
 
 class ServeConfig(ctypes.Structure):
@@ -285,21 +285,22 @@ def _pack(positions, depths):
 
 
 def perft_batch(positions, depths, leaf_plies: int = 0, grid_per_cu: int = 0, use_graph: bool = True,
-                use_tt: bool = False) -> list[int]:
+                use_tt: bool = False, tt_budget_mb: int = 0) -> list[int]:
+    """Batched perft; 0 for leaf_plies, grid_per_cu and tt_budget_mb means autotuned."""
     n, arr, d = _pack(positions, depths)
     out = (c_uint64 * max(n, 1))()
-    opt = BatchOptions(grid_per_cu, leaf_plies, int(use_graph), int(use_tt))
+    opt = BatchOptions(grid_per_cu, leaf_plies, int(use_graph), int(use_tt), tt_budget_mb)
     _check(lib().pg_perft_batch(n, arr, d, byref(opt), out), "pg_perft_batch")
     return list(out[:n])
 
 
 def batch_tuned() -> list[dict]:
     """The batch autotuner's cached choices."""
-    rows = (c_int * (6 * 256))()
+    rows = (c_int * (7 * 256))()
     n = c_int()
     _check(lib().pg_batch_tuned(rows, 256, ctypes.byref(n)), "pg_batch_tuned")
-    keys = ("depth", "size_log2", "tt_requested", "leaf_plies", "tt_used", "grid_per_cu")
-    return [dict(zip(keys, rows[6 * i:6 * i + 6])) for i in range(n.value)]
+    keys = ("depth", "size_log2", "tt_requested", "leaf_plies", "tt_used", "grid_per_cu", "tt_span_log2")
+    return [dict(zip(keys, rows[7 * i:7 * i + 7])) for i in range(n.value)]
 
 
 def perft_batch_plain(positions, depths, use_tt: bool = False) -> list[int]:

@@ -23,9 +23,10 @@
 //    merges identical positions across all roots and counts only what is left
 //    to expand; the TT leaf visits its chunks in a fixed permuted order and an
 //    upsweep stores every merged subtree once;
-//  * an autotuner picks the leaf plies (1 or 2) and, with the table on,
-//    whether a group uses it, per (depth, group size), by timing each choice
-//    on a sample of the group, and caches the choice.
+//  * an autotuner picks the leaf plies (1 or 2), with the table on whether a
+//    group uses it and how much of it, and the persistent grid, per (depth,
+//    group size), by timing each choice on a sample of the group, and caches
+//    the choice.
 #include <string.h>
 
 #include <algorithm>
@@ -70,6 +71,7 @@ struct BatchDev {
     TTEntry *tt;             // shared device table (TT mode)
     u64 ttMask;
     u32 depth, useTT;
+    u64 ttSalt;              // XORed into every table key (0 = the shared key space)
     u64 ttCtr[3];            // probes, hits, stores
     LvlDev lv[MAXLV];
 };
@@ -84,6 +86,17 @@ __device__ __forceinline__ void *dalloc(BatchDev *d, u64 bytes)
     void *p = d->arena + d->arenaOff;
     d->arenaOff += bytes;
     return p;
+}
+
+// A table key in the group's key space.  The autotuner gives each trial its
+// own salt, so that a trial never finds what an earlier trial stored for the
+// same sample; a salted entry has its real depth and count, and no unsalted
+// probe can match it.
+__device__ __forceinline__ Hash128 salted(Hash128 h, u64 salt)
+{
+    h.lo ^= salt;
+    h.hi ^= salt;
+    return h;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -108,6 +121,7 @@ __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
     TTEntry *const tt = d->tt;
     const u64 ttMask = d->ttMask;
     const int depth = (int)d->depth;
+    const u64 salt = d->ttSalt;
     TTTally tally;
     u64 tiles = (a.n + BS - 1) / BS;
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
@@ -117,7 +131,7 @@ __global__ __launch_bounds__(BS) void k_b_count0(BatchDev *d)
         if (i < a.n) {
             if (useTT) {
                 u64 v;
-                hit = ttProbe(tt, ttMask, hashPosition(a.boards[i], a.states[i], WHITE), depth, &v);
+                hit = ttProbe(tt, ttMask, salted(hashPosition(a.boards[i], a.states[i], WHITE), salt), depth, &v);
                 a.known[i] = hit ? v + 1 : 0;
                 tally.add(true, hit, false);
             }
@@ -143,6 +157,7 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
         TTEntry *const tt = d->tt;
         const u64 ttMask = d->ttMask;
         const int rd = (int)d->depth - L;
+        const u64 salt = d->ttSalt;
         TTTally tally;
         for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
             u32 pi = a.idx[i];
@@ -152,7 +167,7 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
             a.boards[i] = q;
             a.states[i] = st;
             u64 v;
-            bool hit = ttProbe(tt, ttMask, hashPosition(q, st, 1 - C), rd, &v);
+            bool hit = ttProbe(tt, ttMask, salted(hashPosition(q, st, 1 - C), salt), rd, &v);
             a.known[i] = hit ? v + 1 : 0;
             tally.add(true, hit, false);
         }
@@ -562,6 +577,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
     const u64 n = a.n;
     TTEntry *const tt = d->tt;
     const u64 ttMask = d->ttMask;
+    const u64 salt = d->ttSalt;
     TTTally tally;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // chunk k of each wave's sequence is chunk k * P mod nChunks of the level
@@ -581,7 +597,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
             makeMove<C>(q, st, a.mv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            hit = ttProbe(tt, ttMask, hashPosition(q, st, 1 - C), 2, &hv);
+            hit = ttProbe(tt, ttMask, salted(hashPosition(q, st, 1 - C), salt), 2, &hv);
             cnt = hit ? 0 : countChildren<1 - C>(q, st);
         }
         if (lane < B_CHUNK) sacc[w][lane] = 0;
@@ -616,7 +632,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
         if (valid) {
             u64 v = hit ? hv : sacc[w][lane];
             a.val[e] = v;
-            if (store) ttStore(tt, ttMask, hashPosition(sb[w][lane], ss[w][lane], 1 - C), 2, v);
+            if (store) ttStore(tt, ttMask, salted(hashPosition(sb[w][lane], ss[w][lane], 1 - C), salt), 2, v);
         }
         tally.add(valid, hit, store);
         waveLdsSync();
@@ -633,6 +649,7 @@ __global__ __launch_bounds__(BS) void k_b_upsweep(BatchDev *d, int L)
     const int rd = (int)d->depth - L;
     TTEntry *const tt = d->tt;
     const u64 ttMask = d->ttMask;
+    const u64 salt = d->ttSalt;
     TTTally tally;
     for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
         if (a.redir[i] != (u32)i) continue;  // duplicates are filled in by k_b_redirect
@@ -643,7 +660,7 @@ __global__ __launch_bounds__(BS) void k_b_upsweep(BatchDev *d, int L)
             v = 0;
             u64 b = i ? a.cnt[i - 1] : 0, e = a.cnt[i];
             for (u64 k = b; k < e; k++) v += cv[k];
-            ttStore(tt, ttMask, hashPosition(a.boards[i], a.states[i], C), rd, v);
+            ttStore(tt, ttMask, salted(hashPosition(a.boards[i], a.states[i], C), salt), rd, v);
             tally.add(false, false, true);
         }
         a.val[i] = v;
@@ -659,6 +676,7 @@ __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
     TTEntry *const tt = d->tt;
     const u64 ttMask = d->ttMask;
     const int depth = (int)d->depth;
+    const u64 salt = d->ttSalt;
     u64 *const rootCounts = d->rootCounts;
     for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i < a.n; i += (u64)gridDim.x * BS) {
         u64 v;
@@ -668,7 +686,7 @@ __global__ __launch_bounds__(BS) void k_b_rootsum(BatchDev *d)
             v = 0;
             u64 b = i ? a.cnt[i - 1] : 0, e = a.cnt[i];
             for (u64 k = b; k < e; k++) v += cv[k];
-            ttStore(tt, ttMask, hashPosition(a.boards[i], a.states[i], WHITE), depth, v);
+            ttStore(tt, ttMask, salted(hashPosition(a.boards[i], a.states[i], WHITE), salt), depth, v);
         }
         rootCounts[i] = v;
     }
@@ -679,14 +697,33 @@ struct GroupPlan {
     int depth, plies, leafLevel;
     bool useTT;
     int gridPerCU = 8;  // blocks per CU of the persistent level kernels and the TT leaf
+    int ttLog2 = 0;     // the table's first 2^ttLog2 entries, 0 = the whole table
+    u64 ttSalt = 0;     // key space of the group's table entries (autotuner trials)
 };
 
-// a tuned choice: leaf plies, table use, persistent grid
+// a tuned choice: leaf plies, table use, persistent grid, table span
 struct TunedPlan {
     int plies;
     bool useTT;
     int gridPerCU;
+    int ttLog2 = 0;
 };
+
+// mask of the table span a group probes and stores in: its first 2^log2
+// entries (every entry is keyed by the full hash, so any span is exact)
+static u64 ttSpanMask(int log2)
+{
+    const u64 full = lib().tt.mask;
+    if (log2 <= 0 || log2 >= 63) return full;
+    const u64 m = (1ull << log2) - 1;
+    return m < full ? m : full;
+}
+static int log2Floor(u64 v)
+{
+    int r = 0;
+    while (v > 1) v >>= 1, r++;
+    return r;
+}
 
 // One batched engine: its descriptor and graphs, and where it runs.  The
 // library's own (g_batch) runs on the library arena and stream under the
@@ -822,7 +859,8 @@ static int runGroup(BatchState &B, const GroupPlan &g, const QuadBitBoard *roots
     h.depth = (u32)g.depth;
     h.useTT = g.useTT ? 1 : 0;
     h.tt = (TTEntry *)L.tt.entries;
-    h.ttMask = L.tt.mask;
+    h.ttMask = ttSpanMask(g.ttLog2);
+    h.ttSalt = g.ttSalt;
     PG_HIP(hipMemcpyAsync(B.dev, &h, sizeof h, hipMemcpyHostToDevice, st));
     // the event profiler is single-threaded: only the library's engine records
     Profiler &P = L.prof;
@@ -878,7 +916,7 @@ static int runGroupSplit(BatchState &B, const GroupPlan &g, const QuadBitBoard *
             flipColors(kids[i], ks[i]);
         }
         int plies = g.plies < g.depth - 1 ? g.plies : g.depth - 2;
-        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies, g.useTT, g.gridPerCU};
+        GroupPlan sub{g.depth - 1, plies, g.depth - 1 - plies, g.useTT, g.gridPerCU, g.ttLog2, g.ttSalt};
         std::vector<u64> kc(nc);
         if (nc) PG_TRY(runGroupSplit(B, sub, kids.data(), ks.data(), nc, useGraph, kc.data()));
         u64 total = 0;
@@ -893,18 +931,25 @@ static int runGroupSplit(BatchState &B, const GroupPlan &g, const QuadBitBoard *
 
 // Autotuner: picks a group's leaf plies (1 or 2: where the BFS hands over to
 // the fused leaf), when the table is on whether the group uses it (its probes
-// and stores cost more than they save on shallow groups), and then the
+// and stores cost more than they save on shallow groups), then the
 // persistent grid of its level kernels and TT leaf (4, 8 or 16 blocks per
-// CU), by timing each choice on a sample of the group.  Each choice first
+// CU), then how much of the device table it uses: the whole table, or its
+// first 1 GiB or 128 MiB (a span that the 256 MB last-level cache can hold
+// makes probes cheaper, and a span too small for the group's stores loses
+// hits), by timing each choice on a sample of the group.  Each choice first
 // runs on one sample (graph capture, first launches, table warm-up) and is
 // timed on a second, disjoint one when the group has room, so a timed run
-// does not hit its own entries.  The sample shrinks 4x per ply above depth 4
-// so that tuning a deep group stays cheap.  Choices are cached per (depth,
-// log2 size, table requested).
+// does not hit its own entries.  A span is tried only when it holds at least
+// eight entries per store the whole group is expected to make (the timed
+// sample's stores scaled to the group).  The sample shrinks 4x per ply above
+// depth 4 so that tuning a deep group stays cheap.  Choices are cached per
+// (depth, log2 size, table requested).
 constexpr int TUNE_MAX_DEPTH = 7;  // groups this deep are not tuned
 static const int TUNE_GRIDS[] = {4, 8, 16};
+static const int TUNE_SPANS[] = {26, 23};  // log2 entries: 1 GiB and 128 MiB of 16-B entries
+static u64 g_tuneTrials = 0;               // trials run so far (their key salts)
 static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roots, const std::vector<u8> &states,
-                                 bool useGraph, bool useTT, int forcedGrid)
+                                 bool useGraph, bool useTT, int forcedGrid, int forcedSpan)
 {
     int bucket = 0;
     for (u64 n = roots.size(); n > 1; n >>= 1) bucket++;
@@ -914,35 +959,52 @@ static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roo
     // Deep groups are leaf-bound, where the fused two-ply leaf always wins
     // and the table always pays; a one-ply trial there would cost more than
     // a whole shallow batch (startpos children at depth 9: minutes).
-    if (depth >= TUNE_MAX_DEPTH) return g_batch.tuned[key] = {2, useTT, forcedGrid ? forcedGrid : 8};
+    if (depth >= TUNE_MAX_DEPTH) return g_batch.tuned[key] = {2, useTT, forcedGrid ? forcedGrid : 8, forcedSpan};
     u64 ns = 256;
     for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
     ns = std::min<u64>(roots.size(), ns);
     const u64 t0i = roots.size() >= 2 * ns ? ns : 0;  // timed sample [t0i, t0i + ns)
     std::vector<u64> tmp(ns);
-    auto trial = [&](const GroupPlan &g) {
+    u64 stores = 0;  // table stores of the last timed run
+    auto trial = [&](GroupPlan g) {
+        // a key space of its own: no trial finds what another stored
+        g.ttSalt = 0x9E3779B97F4A7C15ULL * ++g_tuneTrials;
         double t = 1e30;
         if (runGroupSplit(g_batch, g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
+            uint64_t before[6], after[6];
+            ttStats(before);
             auto a = std::chrono::steady_clock::now();
             if (runGroupSplit(g_batch, g, roots.data() + t0i, states.data() + t0i, ns, useGraph, tmp.data()) == PG_OK)
                 t = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+            ttStats(after);
+            stores = after[2] - before[2];
         }
         return t;
     };
     const int grid0 = forcedGrid ? forcedGrid : 8;
     double best = 1e30;
-    TunedPlan choice{2, useTT, grid0};
+    TunedPlan choice{2, useTT, grid0, forcedSpan};
+    u64 choiceStores = 0;
     for (int tt = useTT ? 1 : 0; tt >= 0; tt--) {
         for (int p = 1; p <= 2 && p < depth; p++) {
-            double t = trial(GroupPlan{depth, p, depth - p, tt != 0, grid0});
-            if (t < best) best = t, choice = {p, tt != 0, grid0};
+            double t = trial(GroupPlan{depth, p, depth - p, tt != 0, grid0, forcedSpan});
+            if (t < best) best = t, choice = {p, tt != 0, grid0, forcedSpan}, choiceStores = stores;
         }
     }
     if (!forcedGrid) {
         for (int gpc : TUNE_GRIDS) {
             if (gpc == grid0) continue;
-            double t = trial(GroupPlan{depth, choice.plies, depth - choice.plies, choice.useTT, gpc});
+            double t = trial(GroupPlan{depth, choice.plies, depth - choice.plies, choice.useTT, gpc, forcedSpan});
             if (t < best) best = t, choice.gridPerCU = gpc;
+        }
+    }
+    if (choice.useTT && !forcedSpan) {
+        const double groupStores = (double)choiceStores * (double)roots.size() / (double)ns;
+        const int full = log2Floor(lib().tt.mask + 1);
+        for (int span : TUNE_SPANS) {
+            if (span >= full || (double)(1ull << span) < 8.0 * groupStores) continue;
+            double t = trial(GroupPlan{depth, choice.plies, depth - choice.plies, true, choice.gridPerCU, span});
+            if (t < best) best = t, choice.ttLog2 = span;
         }
     }
     g_batch.tuned[key] = choice;
@@ -1021,6 +1083,8 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
     bool useTT = opt && opt->useTT && L.tt.entries;
     int forcedPlies = opt ? opt->leafPlies : 0;
     const int forcedGrid = opt && opt->gridPerCU > 0 ? (opt->gridPerCU > 32 ? 32 : opt->gridPerCU) : 0;
+    // a table budget in MB: the largest power-of-two span of 16-B entries within it
+    const int forcedSpan = opt && opt->ttBudgetMB > 0 ? log2Floor((u64)opt->ttBudgetMB) + 16 : 0;
     // group by depth; depth <= 1 on the host as the reference does (perft_kernels.cu:623-626)
     std::map<int, std::vector<int>> groups;
     for (int i = 0; i < n; i++) {
@@ -1050,14 +1114,16 @@ int perftBatch(int n, const pg_position *pos, const uint8_t *depths, const pg_ba
         int plies = forcedPlies;
         bool groupTT = useTT;
         int grid = forcedGrid ? forcedGrid : 8;
+        int span = forcedSpan;
         if (plies != 1 && plies != 2) {
-            TunedPlan c = chooseGroupPlan(depth, roots, states, useGraph, useTT, forcedGrid);
+            TunedPlan c = chooseGroupPlan(depth, roots, states, useGraph, useTT, forcedGrid, forcedSpan);
             plies = c.plies;
             groupTT = c.useTT;
             grid = forcedGrid ? forcedGrid : c.gridPerCU;
+            span = forcedSpan ? forcedSpan : c.ttLog2;
         }
         if (plies >= depth) plies = depth - 1;
-        GroupPlan g{depth, plies, depth - plies, groupTT, grid};
+        GroupPlan g{depth, plies, depth - plies, groupTT, grid, span};
         std::vector<u64> out(ids.size());
         PG_TRY(runGroupSplit(g_batch, g, roots.data(), states.data(), ids.size(), useGraph, out.data()));
         for (size_t k = 0; k < ids.size(); k++) counts[ids[k]] = out[k];
@@ -1082,13 +1148,14 @@ int pg_batch_tuned(int *rows, int maxRows, int *nOut)
     int n = 0;
     for (auto &kv : g_batch.tuned) {
         if (n >= maxRows) break;
-        int *r = rows + 6 * n++;
+        int *r = rows + 7 * n++;
         r[0] = std::get<0>(kv.first);
         r[1] = std::get<1>(kv.first);
         r[2] = std::get<2>(kv.first);
         r[3] = kv.second.plies;
         r[4] = kv.second.useTT ? 1 : 0;
         r[5] = kv.second.gridPerCU;
+        r[6] = kv.second.useTT && lib().tt.entries ? log2Floor(ttSpanMask(kv.second.ttLog2) + 1) : 0;
     }
     *nOut = n;
     return PG_OK;

@@ -215,6 +215,28 @@ def test_batch_grid_choices(gpu):
     assert tuned and all(t["grid_per_cu"] in (4, 8, 16) and t["leaf_plies"] in (1, 2) for t in tuned), tuned
 
 
+def test_batch_table_budget(gpu):
+    # the part of the device table a group uses: 1 MB (65,536 entries, far
+    # fewer than the stores, so most are evicted), 128 MB, and the tuner's
+    # choice (the whole 512 MB table or its first 128 MiB) give the golden
+    # counts, and so does a plain call afterwards on the table they filled
+    rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (4, 5)][:300]
+    ps = [gpu.parse_fen(r["fen"]) for r in rows]
+    ds = [r["depth"] for r in rows]
+    want = [r["count"] for r in rows]
+    gpu.tt_init(512, 0)
+    try:
+        for mb in (1, 128, 0):
+            assert gpu.perft_batch(ps, ds, use_tt=True, tt_budget_mb=mb) == want, mb
+        assert gpu.perft_batch(ps, ds, use_tt=True, tt_budget_mb=1) == want  # on a filled table
+        tuned = [t for t in gpu.batch_tuned() if t["tt_requested"] and t["tt_used"]]
+        assert all(t["tt_span_log2"] in (23, 25) for t in tuned), tuned
+        for r, p in zip(rows[:8], ps[:8]):
+            assert gpu.perft(p, r["depth"], launch_depth=r["depth"], use_tt=True) == r["count"]
+    finally:
+        gpu.tt_free()
+
+
 def test_batch_graph_replay_and_eager_agree(gpu):
     rows = [r for r in load_jsonl("c3.jsonl") if r["depth"] in (3, 4)][:600]
     ps = [gpu.parse_fen(r["fen"]) for r in rows]
