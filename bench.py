@@ -51,6 +51,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_TABLE_STEPS = 3  # untimed steps with every kernel timed, after the timed ones
 
 
 def load_fixture(name):
@@ -540,7 +541,11 @@ def main():
         c = pg.perft_bfs(pos, depth)
         assert c == want, (c, want)
         print(f"[bench] warmup {i} ok", file=sys.stderr, flush=True)
+    # the timed steps time only the leaf (the dominant kernel) with HIP
+    # events: every timed launch adds two stream events, about 2 % of a step
+    # when all of a step's 21 launches carry them
     pg.profile_reset()
+    pg.profile_kernels(["leaf1", "leaf2"])
     pg.profile_enable(True)
     pg.device_synchronize()
     t0 = time.perf_counter()
@@ -551,7 +556,19 @@ def main():
     pg.device_synchronize()
     t1 = time.perf_counter()
     pg.profile_enable(False)
+    leaf_stats = pg.profile_get()
+    # then every kernel, on untimed steps, for the per-kernel table and the
+    # level kernels' roofline at this workload's sizes
+    pg.profile_reset()
+    pg.profile_kernels(None)
+    pg.profile_enable(True)
+    for i in range(KERNEL_TABLE_STEPS):
+        assert pg.perft_bfs(pos, depth) == want
+    pg.profile_enable(False)
     stats = pg.profile_get()
+    for k in ("leaf1", "leaf2"):
+        stats.pop(k, None)
+    stats.update(leaf_stats)
     for k, v in stats.items():
         v["name"] = k
     dt = t1 - t0

@@ -235,11 +235,19 @@ typedef struct pg_kernel_stat {
  * 1 v_bitop3_b32, 2 v_lshlrev_b64, 3 v_bcnt_u32_b32, 4 v_lshl_add_u64, 5 the
  * move counter's mix (bitwise, 64-bit shift, select, add, popcount),
  * 6 v_lshlrev_b32, 7 v_alignbit_b32, 8 v_cndmask_b32, 9 v_add_u32,
- * 10 v_and_or_b32, 11 v_sub_co_u32.  *ms is
+ * 10 v_and_or_b32, 11 v_sub_co_u32, 12 v_and_b32, 13 v_or_b32, 14 v_mov_b32,
+ * 15 v_bfrev_b32, 16 v_or3_b32, 17 v_ffbl_b32, 18 v_add3_u32,
+ * 19 v_cmp_eq_u32, 20 v_xor_b32 in its VOP3 encoding.  *ms is
  * the launch's time, *waveInsts the VALU instructions it issued: their
  * quotient is the VALU issue rate the leaf kernels are measured against. */
 int pg_valu_probe(int kind, int wavesPerSimd, uint64_t iters, double *ms, uint64_t *waveInsts);
 int pg_profile_enable(int on);
+/* which kernels an enabled profiler times: bit k for the k-th kernel that
+ * pg_profile_get names (makecount, scan, expand, genmoves, leaf1, leaf2,
+ * batch_group, tt); all by default.  Each timed launch adds two stream
+ * events, so timing only the dominant kernel keeps a timed run closest to
+ * an untimed one. */
+int pg_profile_kernels(uint32_t mask);
 int pg_profile_reset(void);
 int pg_profile_get(pg_kernel_stat *out, int maxOut, int *nOut);
 int pg_device_synchronize(void);

@@ -130,6 +130,7 @@ _SIGS = [
     ("pg_clock", c_double, []),
     ("pg_valu_probe", c_int, [c_int, c_int, c_uint64, POINTER(c_double), POINTER(c_uint64)]),
     ("pg_profile_enable", c_int, [c_int]),
+    ("pg_profile_kernels", c_int, [c_uint32]),
     ("pg_profile_reset", c_int, []),
     ("pg_profile_get", c_int, [POINTER(KernelStat), c_int, POINTER(c_int)]),
     ("pg_device_synchronize", c_int, []),
@@ -335,6 +336,15 @@ def valu_probe(kind: int, waves_per_simd: int, iters: int) -> tuple[float, int]:
 
 def profile_enable(on: bool = True) -> None:
     lib().pg_profile_enable(int(on))
+
+
+PROFILED_KERNELS = ("makecount", "scan", "expand", "genmoves", "leaf1", "leaf2", "batch_group", "tt")
+
+
+def profile_kernels(names=None) -> None:
+    """Time only the named kernels when profiling is on (all when None)."""
+    mask = 0xFFFFFFFF if names is None else sum(1 << PROFILED_KERNELS.index(n) for n in names)
+    _check(lib().pg_profile_kernels(mask), "pg_profile_kernels")
 
 
 def profile_reset() -> None:

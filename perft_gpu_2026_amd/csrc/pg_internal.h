@@ -69,6 +69,7 @@ static const char *const KERNEL_NAMES[K_COUNT] = {"makecount", "scan", "expand",
                                                   "leaf1",     "leaf2", "batch_group", "tt"};
 struct Profiler {
     bool on = false;
+    uint32_t mask = ~0u;  // kernels timed when on (bit = KernelId)
     struct Rec {
         int id;
         hipEvent_t a, b;
@@ -98,7 +99,7 @@ struct Profiler {
     }
     void begin(int id, hipStream_t st)
     {
-        if (!on) return;
+        if (!on || !((mask >> id) & 1)) return;
         cur = id;
         curA = get();
         (void)hipEventRecord(curA, st);

@@ -1144,6 +1144,12 @@ extern "C" int pg_profile_enable(int on)
     return PG_OK;
 }
 
+extern "C" int pg_profile_kernels(uint32_t mask)
+{
+    lib().prof.mask = mask;
+    return PG_OK;
+}
+
 extern "C" int pg_profile_reset(void)
 {
     lib().prof.reset();
