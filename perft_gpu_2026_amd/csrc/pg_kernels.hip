@@ -1217,18 +1217,26 @@ extern "C" void pg_set_leaf_grid(int blocksPerCU) { g_leafBlocksPerCU = blocksPe
 // slider-attack method of the fused two-ply leaf: 0 = computed fills (the
 // default), 1 = fancy magic tables in device memory, 2 = the same with the
 // bishop sets and the square rows in each block's LDS.  The tables are built
-// (and checked against a ray walk for every blocker subset) and uploaded on
-// the first call that asks for them.
+// (and checked against a ray walk for every blocker subset) and uploaded to
+// the library's device by the first call there that asks for them; the mode
+// applies to the launches that follow (a call made while a server runs is
+// refused, so no kernel in flight sees it change).
 extern "C" int pg_set_slider_attacks(int mode)
 {
     if (mode < SL_FILLS || mode > SL_MAGIC_LDS) {
         setError("pg_set_slider_attacks: mode %d is not 0, 1 or 2", mode);
         return PG_ERR_INVALID;
     }
+    Lib &L = lib();
+    if (!L.init) {
+        setError("pg_init has not run");
+        return PG_ERR_NOT_INIT;
+    }
+    std::lock_guard<std::mutex> lock(L.mu);
+    PG_TRY(refuseWhileServing("pg_set_slider_attacks"));
     if (mode != SL_FILLS) {
-        static bool uploaded = false;
-        if (!lib().init) return PG_ERR_NOT_INIT;
-        if (!uploaded) {
+        static int uploadedTo = -1;  // the device that holds the tables
+        if (uploadedTo != L.device) {
             std::unique_ptr<MagicTables> t(new MagicTables);
             if (!buildMagicTables(*t)) {
                 setError("pg_set_slider_attacks: the magic search failed");
@@ -1236,7 +1244,7 @@ extern "C" int pg_set_slider_attacks(int mode)
             }
             PG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_magicSq), t->sq, sizeof t->sq));
             PG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_magicAtt), t->att, sizeof t->att));
-            uploaded = true;
+            uploadedTo = L.device;
         }
     }
     g_sliderMode = mode;
