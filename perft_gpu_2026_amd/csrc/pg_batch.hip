@@ -195,16 +195,8 @@ __global__ __launch_bounds__(BS) void k_b_makecount(BatchDev *d, int L)
 
 // TT mode, levels >= 1: merge identical positions of level L (the scheme of
 // the plain engine's k_dedup_*), then count the children of the entries that
-// remain to be expanded.  The table is cleared by a kernel, not a memset node,
-// so the captured graph replays correctly.
-__global__ __launch_bounds__(BS) void k_b_dedup_clear(BatchDev *d, int L)
-{
-    if (d->oom) return;
-    const LvlDev a = d->lv[L];
-    for (u64 i = (u64)blockIdx.x * BS + threadIdx.x; i <= a.dmask; i += (u64)gridDim.x * BS)
-        a.dtab[i] = DedupEntry{0, 0};
-}
-
+// remain to be expanded.  The table is not cleared: every slot that
+// k_b_dedup_count reads was written by its reader in k_b_dedup_write.
 template <int C>  // side to move in level L's boards
 __global__ __launch_bounds__(BS) void k_b_dedup_write(BatchDev *d, int L)
 {
@@ -763,7 +755,6 @@ static int enqueueGroup(BatchState &B, const GroupPlan &g, hipStream_t st)
             if (side == BLACK) hipLaunchKernelGGL(k_b_makecount<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
             else hipLaunchKernelGGL(k_b_makecount<BLACK>, dim3(G), dim3(BS), 0, st, d, L);
             if (g.useTT) {
-                hipLaunchKernelGGL(k_b_dedup_clear, dim3(G), dim3(BS), 0, st, d, L);
                 if (side == WHITE) {
                     hipLaunchKernelGGL(k_b_dedup_write<WHITE>, dim3(G), dim3(BS), 0, st, d, L);
                     hipLaunchKernelGGL(k_b_dedup_count<WHITE>, dim3(G), dim3(BS), 0, st, d, L);

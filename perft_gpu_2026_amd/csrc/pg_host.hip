@@ -275,7 +275,7 @@ int initLib(int device, uint64_t arenaBytes)
     arenaBytes &= ~(uint64_t)255;
     PG_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     PG_HIP(hipMalloc(&L.arena, arenaBytes));
-    PG_HIP(hipHostMalloc((void **)&L.hostPinned, 4096, hipHostMallocDefault));
+    PG_HIP(hipHostMalloc((void **)&L.hostPinned, PINNED_BYTES, hipHostMallocDefault));
     // the device address of this GPU's static scan workspace (pg_scan_u32_static)
     PG_TRY(initStaticScratch(device));
     L.arenaBytes = arenaBytes;
@@ -466,7 +466,7 @@ static int frontierPerftParallel(const pg_position *pos, int depth, int ld, u128
         PG_TRY(refuseWhileServing("GPU perft"));
         for (auto &s : g_ttSlots) {
             if (!s.stream) PG_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-            if (!s.pinned) PG_HIP(hipHostMalloc((void **)&s.pinned, 4096, hipHostMallocDefault));
+            if (!s.pinned) PG_HIP(hipHostMalloc((void **)&s.pinned, PINNED_BYTES, hipHostMallocDefault));
         }
         const bool profOn = L.prof.on;
         L.prof.on = false;  // the event profiler is single-threaded

@@ -136,6 +136,12 @@ struct Profiler {
     }
 };
 
+// The pinned host buffer of every stream that runs GPU BFS calls (the
+// library's, each serving and frontier slot's): its first bytes take the
+// readbacks, and bytes [PIN_STAGE, PINNED_BYTES) stage a call's inputs for
+// its one host-to-device copy.
+constexpr size_t PINNED_BYTES = 4096, PIN_STAGE = 1024;
+
 struct Lib {
     Profiler prof;
     bool init = false;

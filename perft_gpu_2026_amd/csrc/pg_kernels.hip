@@ -742,6 +742,39 @@ struct TTLevel {
     u64 n;
 };
 
+// A call's inputs in one host-to-device copy, staged in the stream's pinned
+// buffer: the root board, its state, `nz` zeroed u64 counters, and the first
+// level's parent indices (all the root, 0) and root moves.  The device block
+// is [board 32 B | state, padded to 8 | nz x 8 B | n1 x 4 B | n1 x 2 B].
+struct RootBlock {
+    QuadBitBoard *b0;
+    u8 *s0;
+    u64 *zeros;
+    u32 *idx;
+    u16 *mv;
+};
+static int uploadRoot(Bump &A, const QuadBitBoard &root, u8 state, const u16 *moves, u32 n1, int nz, u64 *pinned,
+                      hipStream_t st, RootBlock &r)
+{
+    const size_t offZ = 40, offIdx = offZ + 8 * (size_t)nz, offMv = offIdx + 4 * (size_t)n1,
+                 bytes = offMv + 2 * (size_t)n1;
+    static_assert(PIN_STAGE + 40 + 8 * 4 + 6 * MAX_MOVES <= PINNED_BYTES, "staging area too small");
+    if (n1 > MAX_MOVES || nz > 4) {
+        setError("uploadRoot: %u root moves, %d counters", n1, nz);
+        return PG_ERR_INVALID;
+    }
+    char *d = A.alloc<char>(bytes);
+    if (!d) return PG_ERR_ARENA_OOM;
+    char *h = (char *)pinned + PIN_STAGE;
+    memcpy(h, &root, sizeof(QuadBitBoard));
+    memset(h + 32, 0, offMv - 32);  // the state's padding, the counters and the parent indices
+    h[32] = (char)state;
+    memcpy(h + offMv, moves, 2 * (size_t)n1);
+    PG_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+    r = {(QuadBitBoard *)d, (u8 *)(d + 32), (u64 *)(d + offZ), (u32 *)(d + offIdx), (u16 *)(d + offMv)};
+    return PG_OK;
+}
+
 static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, const DeviceTT &T,
                       Bump &A, hipStream_t st, u64 *pinned, uint64_t *count)
 {
@@ -754,19 +787,13 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
         *count = 0;
         return PG_OK;
     }
-    QuadBitBoard *b0 = A.alloc<QuadBitBoard>(1);
-    u8 *s0 = A.alloc<u8>(1);
-    u64 *dSum = A.alloc<u64>(1);
-    u64 *ctr = A.alloc<u64>(4);
-    u32 *idx = A.alloc<u32>(n1);
-    u16 *mv = A.alloc<u16>(n1);
-    if (!b0 || !s0 || !dSum || !ctr || !idx || !mv) return PG_ERR_ARENA_OOM;
-    PG_HIP(hipMemcpyAsync(b0, &root, sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
-    PG_HIP(hipMemcpyAsync(s0, &state, 1, hipMemcpyHostToDevice, st));
-    PG_HIP(hipMemcpyAsync(mv, rootMoves, n1 * sizeof(u16), hipMemcpyHostToDevice, st));
-    PG_HIP(hipMemsetAsync(idx, 0, n1 * sizeof(u32), st));
-    PG_HIP(hipMemsetAsync(dSum, 0, sizeof(u64), st));
-    PG_HIP(hipMemsetAsync(ctr, 0, 4 * sizeof(u64), st));
+    RootBlock rb;
+    PG_TRY(uploadRoot(A, root, state, rootMoves, n1, 4, pinned, st, rb));  // [leaf sum, probes, hits, stores]
+    const QuadBitBoard *b0 = rb.b0;
+    const u8 *s0 = rb.s0;
+    u64 *dSum = rb.zeros, *ctr = rb.zeros + 1;
+    u32 *idx = rb.idx;
+    u16 *mv = rb.mv;
     std::vector<TTLevel> lv;
     const QuadBitBoard *pb = b0;
     const u8 *ps = s0;
@@ -792,7 +819,7 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
             return PG_ERR_ARENA_OOM;
         dim3 g(gridFor(n, BS)), gc(tallyGrid(n)), b(BS);
         int rd = depth - lvl;
-        PG_HIP(hipMemsetAsync(tab, 0, tslots * sizeof(DedupEntry), st));
+        // the merge table needs no clearing (every slot read is written first)
         if (pc == WHITE) {
             hipLaunchKernelGGL(k_make_probe<WHITE>, gc, b, 0, st, pb, ps, idx, mv, n, ob, os, known, tt, mask, rd, ctr);
             hipLaunchKernelGGL(k_dedup_write<BLACK>, g, b, 0, st, ob, os, known, n, tab, tslots - 1);
@@ -865,8 +892,7 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
     }
     hipLaunchKernelGGL(k_sum_u64, dim3(gridFor(n1, BS)), dim3(BS), 0, st, childVal, (u64)n1, dSum);
     PG_HIP(hipGetLastError());
-    PG_HIP(hipMemcpyAsync(pinned, dSum, sizeof(u64), hipMemcpyDeviceToHost, st));
-    PG_HIP(hipMemcpyAsync(pinned + 1, ctr, 3 * sizeof(u64), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipMemcpyAsync(pinned, dSum, 4 * sizeof(u64), hipMemcpyDeviceToHost, st));  // the sum, then ctr[0..2]
     PG_HIP(hipStreamSynchronize(st));
     *count = pinned[0];
     ttCountDevice(pinned[1], pinned[2], pinned[3]);
@@ -905,18 +931,14 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
         if (peakBytes) *peakBytes = A.peak;
         return s;
     }
-    QuadBitBoard *b0 = A.alloc<QuadBitBoard>(1);
-    u8 *s0 = A.alloc<u8>(1);
-    u64 *dSum = A.alloc<u64>(2);  // [leaf sum, leaf chunk queue]
-    u32 *idx = A.alloc<u32>(n1);
-    u16 *mv = A.alloc<u16>(n1);
-    if (!b0 || !s0 || !dSum || !idx || !mv) return PG_ERR_ARENA_OOM;
+    RootBlock rb;
+    PG_TRY(uploadRoot(A, root, state, rootMoves, n1, 2, pinned, st, rb));  // [leaf sum, leaf chunk queue]
+    const QuadBitBoard *b0 = rb.b0;
+    const u8 *s0 = rb.s0;
+    u64 *dSum = rb.zeros;
+    u32 *idx = rb.idx;
+    u16 *mv = rb.mv;
     Lib &L = lib();
-    PG_HIP(hipMemcpyAsync(b0, &root, sizeof(QuadBitBoard), hipMemcpyHostToDevice, st));
-    PG_HIP(hipMemcpyAsync(s0, &state, 1, hipMemcpyHostToDevice, st));
-    PG_HIP(hipMemcpyAsync(mv, rootMoves, n1 * sizeof(u16), hipMemcpyHostToDevice, st));
-    PG_HIP(hipMemsetAsync(idx, 0, n1 * sizeof(u32), st));
-    PG_HIP(hipMemsetAsync(dSum, 0, 2 * sizeof(u64), st));
 
     const QuadBitBoard *pb = b0;
     const u8 *ps = s0;

@@ -92,8 +92,12 @@ __device__ __forceinline__ void ttStore(TTEntry *tt, u64 mask, const Hash128 &h,
 // ----------------------------------------------------- within-level merging
 // (write_dedup_table / check_duplicates / apply_redirects,
 // perft_kernels.cu:486-552): every unknown entry of a level writes
-// {hash.hi, index + 1} to a cleared table of at least 2n slots, then reads its
-// slot back; an entry that finds another index with its key AND an identical
+// {hash.hi, index + 1} to a table of at least 2n slots, then (in the next
+// kernel) reads its slot back.  The table is never cleared: every slot that
+// is read was written by the entry reading it, so it holds one of this
+// level's entries (both 8-B halves come from this level's writers), never
+// what an earlier level or call left there.  An entry that finds another
+// index with its key AND an identical
 // position (boards compared, so a hash collision or a torn 16-B slot can
 // never merge two different positions) becomes a duplicate with no children,
 // and takes the original's value after the upsweep.
