@@ -596,8 +596,12 @@ PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state, const SL &sl = SL())
     BB pw = P & (freeP | (C == WHITE ? f.pinAnti : f.pinDiag));
     BB ce = (C == WHITE ? shift<9>(pe) : shift<-7>(pe)) & NOT_A & s.them & f.checkMask;
     BB cw = (C == WHITE ? shift<7>(pw) : shift<-9>(pw)) & NOT_H & s.them & f.checkMask;
-    n += popc(s1 & ~promo) + popc(s2) + popc(ce & ~promo) + popc(cw & ~promo) +
-         4 * (popc(s1 & promo) + popc(ce & promo) + popc(cw & promo));
+    // pushes land on empty squares and captures on enemy ones, and a square
+    // is reached by a single push or by a double push, never both (the double
+    // push needs the single-push square empty): s1, s2 and ce are disjoint,
+    // so one popcount covers them; cw can share squares with ce.  A promotion
+    // counts four moves: three more for each promoting target.
+    n += popc(or3(s1, s2, ce)) + popc(cw) + 3 * (popc((s1 | ce) & promo) + popc(cw & promo));
 
     // castling: rights, empty path, unattacked path, not in check
     u32 cr = C == WHITE ? (state & 3) : ((state >> 2) & 3);
