@@ -135,6 +135,16 @@ PG_HD BB or3(BB a, BB b, BB c)
     return a | b | c;
 #endif
 }
+// (a & b) != 0: the halves' intersections are or-ed by one v_bitop3_b32
+// (the compiler would form v_and_or_b32, which does not pair)
+PG_HD bool meets(BB a, BB b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a.lo, b.lo, a.hi & b.hi, 0xEA) != 0;
+#else
+    return (bool)(a & b);
+#endif
+}
 PG_HD BB rev64(BB x) { return BB(__builtin_bitreverse32(x.hi), __builtin_bitreverse32(x.lo), 0); }
 
 // Board shift by S (positive: towards h8).  On the device one 64-bit shift
@@ -395,7 +405,7 @@ PG_HD Frame framePins(const QuadBitBoard &q)
         int sq = lsb(sb);
         // the sniper's line, selected per 32-bit half (a select of whole BB
         // values here is lowered to an indexed private array)
-        bool inF = (bool)(mFile & sb), inR = (bool)(mRank & sb), inD = (bool)(mDiag & sb);
+        bool inF = meets(mFile, sb), inR = meets(mRank, sb), inD = meets(mDiag, sb);
         BB line(inF ? mFile.lo : inR ? mRank.lo : inD ? mDiag.lo : mAnti.lo,
                 inF ? mFile.hi : inR ? mRank.hi : inD ? mDiag.hi : mAnti.hi, 0);
         bool up = sq > k;
@@ -403,7 +413,7 @@ PG_HD Frame framePins(const QuadBitBoard &q)
         BB blk = btw & s.occ;
         BB ray = btw | sb;
         bool check = !blk;
-        bool pin = blk && !clearLow(blk) && (blk & s.us);
+        bool pin = blk && !clearLow(blk) && meets(blk, s.us);
         chkRays |= check ? ray : BB();
         behind |= check ? (f.kAtt & line & (up ? below : above)) : BB();
         pinAll |= pin ? ray : BB();
@@ -472,7 +482,7 @@ PG_HD BB epSources(const Frame &f, u8 state)
     }
     // a pinned pawn captures only along its own pin line: the target must lie
     // on the pin rays of the diagonal (or anti-diagonal) the pawn is pinned on
-    BB onLine = ((target & f.pinDiag) ? f.pinDiag : BB()) | ((target & f.pinAnti) ? f.pinAnti : BB());
+    BB onLine = (meets(target, f.pinDiag) ? f.pinDiag : BB()) | (meets(target, f.pinAnti) ? f.pinAnti : BB());
     BB legal = src & f.pinned & onLine;
     BB freeSrc = src & ~f.pinned;
     // removing both pawns must not open the king's rank to a rook/queen
@@ -809,14 +819,13 @@ PG_HD void makeMove(QuadBitBoard &q, u8 &state, u16 m)
     q.bb[2] = andOr(b2, clr, (piece & 2) ? tb : BB()).v();
     q.bb[3] = andOr(b3, clr, ((piece & 4) ? tb : BB()) | rookTo).v();
 
-    u64 touch = fb64 | tb64;
-    u32 c = 0;
-    c |= (touch & (1ULL << 0)) ? 2u : 0u;
-    c |= (touch & (1ULL << 7)) ? 1u : 0u;
-    c |= (touch & (1ULL << 4)) ? 3u : 0u;
-    c |= (touch & (1ULL << 56)) ? 8u : 0u;
-    c |= (touch & (1ULL << 63)) ? 4u : 0u;
-    c |= (touch & (1ULL << 60)) ? 12u : 0u;
+    // castling rights lost when a move touches a1/e1/h1/a8/e8/h8: each right's
+    // two squares shifted onto its bit (K = 1: e1, h1; Q = 2: a1, e1;
+    // k = 4: e8, h8; q = 8: a8, e8), squares 56-63 being bits 24-31 of hi
+    const BB touch = fb | tb;
+    const u32 lo = touch.lo, hi = touch.hi;
+    const u32 c = (((lo >> 4) | (lo >> 7)) & 1u) | (((lo << 1) | (lo >> 3)) & 2u) | (((hi >> 26) | (hi >> 29)) & 4u) |
+                  (((hi >> 21) | (hi >> 25)) & 8u);
     u32 st = state & 0x0F & ~c;
     st |= (flags == MF_DPUSH) ? (((from & 7) + 1) << 4) : 0u;
     state = (u8)st;
