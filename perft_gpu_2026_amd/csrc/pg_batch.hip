@@ -490,7 +490,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2(BatchD
         u32 incl = waveInclScan(cnt);
         u32 off = incl - cnt;
         u32 total = __shfl(incl, 63);
-        u64 acc = 0;
+        u32 acc = 0;  // a lane's pairs of one chunk: at most 218 x 218 moves
         waveLdsSync();
         for (u32 wb = 0; wb < total; wb += BL2_CAP) {
             if (cnt && off < wb + BL2_CAP && off + cnt > wb) {

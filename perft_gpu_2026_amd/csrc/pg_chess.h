@@ -427,7 +427,7 @@ PG_HD Frame framePins(const QuadBitBoard &q)
     f.checkers = andOr(chkRays, eOrth | eDiag, leaperCheckers);
     f.nCheck = popc(f.checkers);
     f.checkMask = f.nCheck == 0 ? BB(~0ULL) : (f.nCheck == 1 ? (chkRays | leaperCheckers) : BB());
-    f.pinned = (or3(f.pinFile, f.pinRank, f.pinDiag) | f.pinAnti) & s.us;
+    f.pinned = pinAll & s.us;  // every pin ray lies on one of the four lines
     // leaper and king attacks here; slider attacks are added by the caller
     BB eK = s.kings & s.them;
     BB eKAtt = eK ? BB(sqMasks(lsb(eK)).king) : BB();

@@ -294,14 +294,16 @@ __device__ __forceinline__ void leaf2Body(const QuadBitBoard *__restrict__ pb, c
             }
             waveLdsSync();
             u32 wn = total - wb < (u32)CAP ? total - wb : (u32)CAP;
+            u32 wacc = 0;  // at most CAP / 64 pairs of at most 218 moves per lane
             for (u32 j = lane; j < wn; j += 64) {
                 u16 x = slm[w][j];
                 u32 src = sll[w][j];
                 QuadBitBoard c = sb[w][src];
                 u8 s2 = ss[w][src];
                 makeMove<1 - C>(c, s2, x);
-                acc += countMovesSL<C>(c, s2, sl);
+                wacc += countMovesSL<C>(c, s2, sl);
             }
+            acc += wacc;
             waveLdsSync();
         }
     }
