@@ -259,9 +259,13 @@ PG_HD constexpr u64 antiMaskEx(int sq)  // a8-h1 direction (NW/SE)
 
 // One 64-byte row per square, built at compile time: the four lines through
 // the square (square excluded), knight and king attacks, and pawn attacks by
-// colour.  On the device the table lives in constant memory (4 KB, cached);
-// a countMoves reads the king's row (lines, king, knight and pawn attacks),
-// the enemy king's king attacks and one row per own knight.
+// colour.  One constexpr table serves host and device: on the device it is
+// read-only data of the code object (4 KB, cached), addressed relative to the
+// program counter.  (A __constant__ table is registered with the runtime for
+// host access and is reached through the GOT: a scalar load, and a wait on
+// it, inside every loop that reads it; this form took the fused leaf 0.25 %
+// lower.)  A countMoves reads the king's row (lines, king, knight and pawn
+// attacks), the enemy king's king attacks and one row per own knight.
 struct SqMasks {
     u64 file, rank, diag, anti, knight, king, pawn[2];
 };
@@ -285,18 +289,11 @@ constexpr SqTable buildSqTable()
     }
     return T;
 }
-#if defined(__HIPCC__)
-static __constant__ const SqTable kSqDev = buildSqTable();
-#endif
 static constexpr SqTable kSqHost = buildSqTable();
 
 PG_HD const SqMasks &sqMasks(int sq)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return kSqDev.t[sq];
-#else
     return kSqHost.t[sq];
-#endif
 }
 
 // ------------------------------------------------------- Kogge-Stone fills
