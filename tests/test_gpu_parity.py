@@ -194,8 +194,11 @@ def _d(gpu, a):
     return gpu.DeviceArray.from_numpy(a)
 
 
-@pytest.mark.parametrize("n", [1, 255, 256, 257, 16_385, 100_000, 3_000_001, 4_194_305, 9_000_000, 67_110_147])
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 16_385, 100_000, 1_048_576, 1_048_577, 3_000_001, 4_194_305,
+                               9_000_000, 67_110_147])
 def test_scan_matches_cumsum(gpu, n):
+    # 1,048,576 entries (64 groups) is the largest level the one-block group
+    # scan takes; one more entry takes the group pass and the top scan
     rng = np.random.default_rng(n)
     host = rng.integers(0, 219, size=n, dtype=np.uint32)
     d = _d(gpu, host)
@@ -206,6 +209,45 @@ def test_scan_matches_cumsum(gpu, n):
     gpu.device_synchronize()
     assert np.array_equal(d.to_numpy(np.uint32, n), np.cumsum(host, dtype=np.uint64).astype(np.uint32))
     assert total.value == int(host.sum(dtype=np.uint64))
+
+
+def test_scan_total_past_u32(gpu):
+    # a total of 2^32 or more: the entries wrap like u32 (the BFS refuses such a
+    # level as out of arena), the 64-bit total is exact
+    n = 20_000_000
+    host = np.full(n, 218, dtype=np.uint32)
+    d = _d(gpu, host)
+    tiles = (n + 255) // 256
+    scratch = gpu.DeviceArray(8 * (tiles + 4) + 4 * tiles)
+    total = ctypes.c_uint64()
+    assert gpu.lib().pg_scan_u32(d.ptr, n, None, scratch.ptr, ctypes.byref(total), None) == 0
+    gpu.device_synchronize()
+    assert total.value == 218 * n > 2**32
+    assert np.array_equal(d.to_numpy(np.uint32, n), np.cumsum(host, dtype=np.uint64).astype(np.uint32))
+
+
+def test_bfs_levels_above_a_million_entries(gpu):
+    # the BFS's large levels (group pass, top scan, apply with partitions): the
+    # config-2 tree with the one-ply leaf scans its perft(5) level (15,143,580
+    # entries), startpos perft 7 its 4,865,609-entry level, and the table BFS
+    # the same levels through its merge pass
+    r = load_jsonl("c2.jsonl")[0]
+    pos = gpu.parse_fen(r["fen"])
+    start = gpu.parse_fen("rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1")
+    gpu.set_leaf_plies(1)
+    try:
+        for d in (6, 7):
+            print(f"[gpu] config-2 perft {d}, one-ply leaf", flush=True)
+            assert gpu.perft_bfs(pos, d) == r["counts"][d - 1]
+        assert gpu.perft_bfs(start, 7) == 3_195_901_860
+        gpu.tt_init(1024, 0)
+        try:
+            print("[gpu] config-2 perft 7 with the table, one-ply leaf", flush=True)
+            assert gpu.perft_bfs(pos, 7, use_tt=True) == r["counts"][6]
+        finally:
+            gpu.tt_free()
+    finally:
+        gpu.set_leaf_plies(2)
 
 
 @pytest.mark.parametrize("n,shift", [(9_000_000, 0), (9_000_000, 1), (700_000, 1), (67_110_147, 1)])
