@@ -414,11 +414,13 @@ __global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
     const LvlDev p = d->lv[L - 1];
     const u64 n = a.n;
     const int lane = threadIdx.x & 63;
+    // next chunk's ticket taken as the current one starts (see k_b_leaf2)
+    u64 ticket = 0;
+    if (lane == 0) ticket = atomicAdd((unsigned long long *)&d->work, 64ull);
     for (;;) {
-        u64 base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long *)&d->work, 64ull);
-        base = __shfl(base, 0);
+        const u64 base = __shfl(ticket, 0);
         if (base >= n) break;
+        if (lane == 0) ticket = atomicAdd((unsigned long long *)&d->work, 64ull);
         u64 e = base + lane;
         bool valid = e < n;
         u32 v = 0, root = 0;
@@ -453,8 +455,8 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2(BatchD
     __shared__ QuadBitBoard sb[BL2_WAVES][B_CHUNK];
     __shared__ u8 ss[BL2_WAVES][B_CHUNK];
     __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
-    __shared__ u16 slm[BL2_WAVES][BL2_CAP];
-    __shared__ u8 sll[BL2_WAVES][BL2_CAP];
+    __shared__ u16 slm[BL2_WAVES][BL2_CAP + 1];  // slot BL2_CAP: moves outside the window
+    __shared__ u8 sll[BL2_WAVES][BL2_CAP + 1];
     const u32 L = d->leafLevel;
     // descriptor fields into registers once (the atomics below would otherwise
     // force them to be re-read from the descriptor)
@@ -466,14 +468,18 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2(BatchD
     u64 *rootCounts = d->rootCounts;
     const u64 n = d->lv[L].n;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the next chunk's queue ticket is taken as the current one starts, so
+    // the atomic's round trip overlaps a chunk's work
+    u64 ticket = 0;
+    if (lane == 0) ticket = atomicAdd((unsigned long long *)&d->work, (unsigned long long)B_CHUNK);
     for (;;) {
-        u64 base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long *)&d->work, (unsigned long long)B_CHUNK);
-        base = __shfl(base, 0);
+        const u64 base = __shfl(ticket, 0);
         if (base >= n) break;
+        if (lane == 0) ticket = atomicAdd((unsigned long long *)&d->work, (unsigned long long)B_CHUNK);
         u32 cnt = 0, root = 0;
         u64 e = base + lane;
         bool valid = lane < B_CHUNK && e < n;
+        BB catt;  // the child's enemy attacks, from its count, for its generation
         if (valid) {
             u32 pi = aIdx[e];
             QuadBitBoard q = pBoards[pi];
@@ -481,7 +487,9 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2(BatchD
             makeMove<C>(q, st, aMv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            cnt = countChildren<1 - C>(q, st);
+            Frame fr;
+            cnt = countChildrenFrame<1 - C>(q, st, FillSliders(), fr);
+            catt = fr.att;
             root = aRoot[e];
         }
         const u32 r0 = __shfl(root, 0);  // lane 0 is always valid
@@ -498,9 +506,12 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2(BatchD
                 u8 st = ss[w][lane];
                 u16 *dm = slm[w];
                 u8 *dl = sll[w];
-                generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+                Frame fr = framePins<1 - C>(q);
+                fr.att = catt;
+                generateMovesF<1 - C>(fr, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;
-                    if (j < BL2_CAP) dm[j] = m, dl[j] = (u8)lane;
+                    j = j < (u32)BL2_CAP ? j : (u32)BL2_CAP;
+                    dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();
@@ -561,8 +572,8 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
     __shared__ QuadBitBoard sb[BL2_WAVES][B_CHUNK];
     __shared__ u8 ss[BL2_WAVES][B_CHUNK];
     __shared__ u32 sacc[BL2_WAVES][B_CHUNK];
-    __shared__ u16 slm[BL2_WAVES][BL2_CAP];
-    __shared__ u8 sll[BL2_WAVES][BL2_CAP];
+    __shared__ u16 slm[BL2_WAVES][BL2_CAP + 1];  // slot BL2_CAP: moves outside the window
+    __shared__ u8 sll[BL2_WAVES][BL2_CAP + 1];
     const u32 L = d->leafLevel;
     const LvlDev a = d->lv[L];
     const LvlDev p = d->lv[L - 1];
@@ -582,6 +593,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
         u32 cnt = 0;
         u64 e = base + lane, hv = 0;
         bool valid = lane < B_CHUNK && e < n, hit = false;
+        BB catt;  // the child's enemy attacks, from its count, for its generation
         if (valid) {
             u32 pi = a.idx[e];
             QuadBitBoard q = p.boards[pi];
@@ -590,7 +602,11 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
             sb[w][lane] = q;
             ss[w][lane] = st;
             hit = ttProbe(tt, ttMask, salted(hashPosition(q, st, 1 - C), salt), 2, &hv);
-            cnt = hit ? 0 : countChildren<1 - C>(q, st);
+            if (!hit) {
+                Frame fr;
+                cnt = countChildrenFrame<1 - C>(q, st, FillSliders(), fr);
+                catt = fr.att;
+            }
         }
         if (lane < B_CHUNK) sacc[w][lane] = 0;
         u32 incl = waveInclScan(cnt);
@@ -603,9 +619,12 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
                 u8 st = ss[w][lane];
                 u16 *dm = slm[w];
                 u8 *dl = sll[w];
-                generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+                Frame fr = framePins<1 - C>(q);
+                fr.att = catt;
+                generateMovesF<1 - C>(fr, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;
-                    if (j < BL2_CAP) dm[j] = m, dl[j] = (u8)lane;
+                    j = j < (u32)BL2_CAP ? j : (u32)BL2_CAP;
+                    dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();

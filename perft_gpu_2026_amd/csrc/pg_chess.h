@@ -530,8 +530,10 @@ struct MagicSliders {
 };
 struct FillSliders {};  // the computed fills below (default)
 
+// With `frameOut` non-null the legality frame, its enemy attacks complete,
+// is handed back for generateMovesF on the same position.
 template <int C, bool EXACT, class SL = FillSliders>
-PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state, const SL &sl = SL())
+PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state, const SL &sl = SL(), Frame *frameOut = nullptr)
 {
     Frame f = framePins<C>(q);
     const Sets &s = f.s;
@@ -587,6 +589,10 @@ PG_HD u32 countMovesT(const QuadBitBoard &q, u8 state, const SL &sl = SL())
     }
 
     n += popc(f.kAtt & ~s.us & ~att);
+    if (frameOut) {
+        *frameOut = f;
+        frameOut->att = att;
+    }
 
     // knights: pinned knights never move
     for (BB N = s.knights & s.us & freeP; N; N = clearLow(N)) n += popc(BB(sqMasks(lsb(N)).knight) & targets);
@@ -639,6 +645,14 @@ PG_HD u32 countChildren(const QuadBitBoard &q, u8 state)
     return countMovesT<C, true>(q, state);
 }
 
+// countChildren that also hands back the position's frame, so that
+// generateMovesF can emit the same moves without recomputing it
+template <int C, class SL>
+PG_HD u32 countChildrenFrame(const QuadBitBoard &q, u8 state, const SL &sl, Frame &f)
+{
+    return countMovesT<C, true, SL>(q, state, sl, &f);
+}
+
 // the same two counts with a chosen slider-attack method (FillSliders or MagicSliders)
 template <int C, class SL>
 PG_HD u32 countMovesSL(const QuadBitBoard &q, u8 state, const SL &sl)
@@ -661,23 +675,68 @@ PG_HD u32 countChildren(const QuadBitBoard &q, u8 state, int color)
 }
 
 // --------------------------------------------------------- move generation
+// Target-set emission.  A set's squares are visited in ascending order, the
+// low 32-bit half first: a move then costs one 32-bit find-first and one
+// clear-lowest (the 64-bit forms take four and four instructions), and the
+// move is one shift-or (pieces: base | to << 6, base = from | flags << 12)
+// or one multiply-add (pawns: from = to - d, so move = 65 to - d + flags).
+PG_HD u32 ctz32(u32 x) { return (u32)__builtin_ctz(x); }
+
+template <typename Emit>
+PG_HD void emitTo(BB t, u32 base, u32 &n, Emit &emit)
+{
+    for (u32 x = t.lo; x; x &= x - 1) emit(n++, (u16)(base | (ctz32(x) << 6)));
+    for (u32 x = t.hi; x; x &= x - 1) emit(n++, (u16)((base | 0x800u) | (ctz32(x) << 6)));
+}
+// pawn moves by d squares (from = to - d), flags already in `add` = flags << 12 - d
+template <typename Emit>
+PG_HD void emitPawnTo(BB t, u32 add, u32 &n, Emit &emit)
+{
+    for (u32 x = t.lo; x; x &= x - 1) emit(n++, (u16)(ctz32(x) * 65u + add));
+    for (u32 x = t.hi; x; x &= x - 1) emit(n++, (u16)((ctz32(x) + 32u) * 65u + add));
+}
+// promotions by d squares, in the order N, B, Q, R (MoveGeneratorBitboard.h:920-923)
+template <typename Emit>
+PG_HD void emitPromoTo(BB t, u32 minusD, u32 &n, Emit &emit)
+{
+    for (BB r = t; r; r = clearLow(r)) {
+        const u32 m = (u32)lsb(r) * 65u + minusD;
+        emit(n, (u16)(m | ((MF_PROMO + 0) << 12)));
+        emit(n + 1, (u16)(m | ((MF_PROMO + 1) << 12)));
+        emit(n + 2, (u16)(m | ((MF_PROMO + 3) << 12)));
+        emit(n + 3, (u16)(m | ((MF_PROMO + 2) << 12)));
+        n += 4;
+    }
+}
+// pawn moves of t by d squares, promotions included, in ascending order of
+// the target: white's promotions (rank 8) follow its other moves, black's
+// (rank 1) come first
+template <int C, typename Emit>
+PG_HD void emitPawnMoves(BB t, int d, u32 &n, Emit &emit)
+{
+    const u64 promo = C == WHITE ? RANK_8 : RANK_1;
+    const BB plain = t & ~BB(promo), pr = t & promo;
+    if constexpr (C == BLACK)
+        if (pr) emitPromoTo(pr, (u32)(-d), n, emit);
+    emitPawnTo(plain, (u32)(-d), n, emit);
+    if constexpr (C == WHITE)
+        if (pr) emitPromoTo(pr, (u32)(-d), n, emit);
+}
+
 // Emits every legal move into `out` (if non-null) and returns the count.
 // Order: king, castling, knights, diagonal sliders, orthogonal sliders,
-// pawn pushes, double pushes, captures, en passant.  A double push carries
-// MF_DPUSH only when an enemy pawn stands beside its target (otherwise no EP
-// state can follow); captures carry no capture flag.
+// pawn pushes, double pushes, captures, en passant; each piece's or pawn
+// set's targets in ascending order.  A double push carries MF_DPUSH only when
+// an enemy pawn stands beside its target (otherwise no EP state can follow);
+// captures carry no capture flag.
 template <int C, typename Emit>
-PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
+PG_HD u32 generateMovesF(const Frame &f, u8 state, Emit &&emit)
 {
-    Frame f = frame<C>(q);
     const Sets &s = f.s;
     BB empty = ~s.occ;
     BB targets = ~s.us & f.checkMask;
     u32 n = 0;
-    for (BB t = f.kAtt & ~s.us & ~f.att; t; t = clearLow(t)) {
-        emit(n, (u16)(f.k | (lsb(t) << 6)));
-        n++;
-    }
+    emitTo(f.kAtt & ~s.us & ~f.att, (u32)f.k, n, emit);
     if (f.nCheck == 0) {
         u32 cr = C == WHITE ? (state & 3) : ((state >> 2) & 3);
         const u64 ksEmpty = C == WHITE ? 0x60ULL : 0x6000000000000000ULL;
@@ -695,76 +754,50 @@ PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
     BB freeP = ~f.pinned;
     for (BB b = s.knights & s.us & freeP; b; b = clearLow(b)) {
         int from = lsb(b);
-        for (BB t = BB(sqMasks(from).knight) & targets; t; t = clearLow(t)) {
-            emit(n, (u16)(from | (lsb(t) << 6)));
-            n++;
-        }
+        emitTo(BB(sqMasks(from).knight) & targets, (u32)from, n, emit);
     }
     for (BB b = s.diag & s.us & ~(f.pinFile | f.pinRank); b; b = clearLow(b)) {
         int from = lsb(b);
         BB fb = bit(from);
         BB t = bishopAttacks(from, s.occ) & targets;
         t &= (fb & f.pinDiag) ? f.pinDiag : ((fb & f.pinAnti) ? f.pinAnti : BB(~0ULL));
-        for (; t; t = clearLow(t)) {
-            emit(n, (u16)(from | (lsb(t) << 6)));
-            n++;
-        }
+        emitTo(t, (u32)from, n, emit);
     }
     for (BB b = s.orth & s.us & ~(f.pinDiag | f.pinAnti); b; b = clearLow(b)) {
         int from = lsb(b);
         BB fb = bit(from);
         BB t = rookAttacks(from, s.occ) & targets;
         t &= (fb & f.pinFile) ? f.pinFile : ((fb & f.pinRank) ? f.pinRank : BB(~0ULL));
-        for (; t; t = clearLow(t)) {
-            emit(n, (u16)(from | (lsb(t) << 6)));
-            n++;
-        }
+        emitTo(t, (u32)from, n, emit);
     }
     BB P = s.pawns & s.us;
-    const u64 promo = C == WHITE ? RANK_8 : RANK_1;
     const int fwd = C == WHITE ? 8 : -8;
     BB push = P & (freeP | f.pinFile);
     BB s1 = (C == WHITE ? shift<8>(push) : shift<-8>(push)) & empty;
     BB s2 = (C == WHITE ? shift<8>(s1 & RANK_3) : shift<-8>(s1 & RANK_6)) & empty & f.checkMask;
     s1 &= f.checkMask;
-    BB ePawns = s.pawns & s.them;
-    auto emitPawn = [&](int from, int to) {
-        if ((1ULL << to) & promo) {
-            // promotion order N, B, Q, R (MoveGeneratorBitboard.h:920-923)
-            emit(n, (u16)(from | (to << 6) | ((MF_PROMO + 0) << 12)));
-            emit(n + 1, (u16)(from | (to << 6) | ((MF_PROMO + 1) << 12)));
-            emit(n + 2, (u16)(from | (to << 6) | ((MF_PROMO + 3) << 12)));
-            emit(n + 3, (u16)(from | (to << 6) | ((MF_PROMO + 2) << 12)));
-            n += 4;
-        } else {
-            emit(n, (u16)(from | (to << 6)));
-            n++;
+    emitPawnMoves<C>(s1, fwd, n, emit);
+    // double pushes: MF_DPUSH where an enemy pawn stands beside the target
+    {
+        BB ePawns = s.pawns & s.them;
+        BB dp = s2 & ((shift<1>(ePawns) & NOT_A) | (shift<-1>(ePawns) & NOT_H));
+        const u32 add = (u32)(-2 * fwd);
+        for (u32 x = s2.lo; x; x &= x - 1) {
+            const u32 to = ctz32(x);
+            emit(n++, (u16)(to * 65u + add + (((dp.lo >> to) & 1u) << 12)));
         }
-    };
-    for (BB t = s1; t; t = clearLow(t)) {
-        int to = lsb(t);
-        emitPawn(to - fwd, to);
-    }
-    for (BB t = s2; t; t = clearLow(t)) {
-        int to = lsb(t);
-        BB tb = bit(to);
-        bool epPossible = (bool)(((shift<1>(tb) & NOT_A) | (shift<-1>(tb) & NOT_H)) & ePawns);
-        emit(n, (u16)((to - 2 * fwd) | (to << 6) | ((epPossible ? MF_DPUSH : MF_QUIET) << 12)));
-        n++;
+        for (u32 x = s2.hi; x; x &= x - 1) {
+            const u32 to = ctz32(x);
+            emit(n++, (u16)((to + 32u) * 65u + add + (((dp.hi >> to) & 1u) << 12)));
+        }
     }
     BB pe = P & (freeP | (C == WHITE ? f.pinDiag : f.pinAnti));
     BB pw = P & (freeP | (C == WHITE ? f.pinAnti : f.pinDiag));
     const int de = C == WHITE ? 9 : -7, dw = C == WHITE ? 7 : -9;
     BB ce = (C == WHITE ? shift<9>(pe) : shift<-7>(pe)) & NOT_A & s.them & f.checkMask;
     BB cw = (C == WHITE ? shift<7>(pw) : shift<-9>(pw)) & NOT_H & s.them & f.checkMask;
-    for (BB t = ce; t; t = clearLow(t)) {
-        int to = lsb(t);
-        emitPawn(to - de, to);
-    }
-    for (BB t = cw; t; t = clearLow(t)) {
-        int to = lsb(t);
-        emitPawn(to - dw, to);
-    }
+    emitPawnMoves<C>(ce, de, n, emit);
+    emitPawnMoves<C>(cw, dw, n, emit);
     if (state >> 4) {
         int to = ((state >> 4) - 1) + (C == WHITE ? 40 : 16);
         for (BB b = epSources<C>(f, state); b; b = clearLow(b)) {
@@ -773,6 +806,12 @@ PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
         }
     }
     return n;
+}
+
+template <int C, typename Emit>
+PG_HD u32 generateMoves(const QuadBitBoard &q, u8 state, Emit &&emit)
+{
+    return generateMovesF<C>(frame<C>(q), state, emit);
 }
 
 struct MoveListSink {
