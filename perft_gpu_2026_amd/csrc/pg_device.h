@@ -6,7 +6,10 @@ namespace pg {
 
 constexpr int BS = 256;          // threads per block: 4 wave64
 constexpr int EXPAND_TILE = 2048;  // children per expand block (8 per thread)
-constexpr int GEN_LCAP = 16384;    // moves staged in LDS per genmoves block (32 KB)
+// moves staged in LDS per genmoves block (20 KB: eight blocks, 32 waves, per
+// CU; a 32 KB tile left five blocks and took the 15-million-board level of
+// the config-2 tree 11 % longer; 16 KB sent too many blocks to direct writes)
+constexpr int GEN_LCAP = 10240;
 
 // ------------------------------------------------------------ wave64 helpers
 __device__ __forceinline__ u32 waveSum(u32 v)
