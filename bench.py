@@ -13,15 +13,20 @@ count.  Inputs live on the device for the whole timed region (the root position
 is 33 bytes; every BFS level is built in HBM).
 
 The JSON line carries:
-  roofline      the dominant kernel (the leaf) against the HBM peak, from
-                live HIP-event timing of every launch on the library stream
-                and the algorithmic bytes 6K + 33P per launch (K leaf entries,
-                P parent boards).  The leaf is VALU-bound integer work, so
-                its HBM fraction is small by nature; `roofline_levels` gives
-                the same for the HBM-bound BFS level kernels (scan, expand) at
-                config 2's own level sizes (small: launch-latency bound), and
-                `roofline_levels_full_size` gives scan and expand on the
-                largest level of the tree (config 2 run with the one-ply leaf).
+  roofline      the dominant kernel (the fused leaf) against its VALU issue
+                slots (bound "valu": a SIMD issues one VALU instruction per
+                four-cycle slot, 1,024 SIMDs x 2.4 GHz / 4 = 614.4 G slots/s):
+                its PMC instruction count per launch, less the second of each
+                dual-issued pair, over the live HIP-event launch time on the
+                library stream.  The same object carries the leaf's HBM
+                figures (algorithmic bytes 6K + 33P per launch, K leaf
+                entries, P parent boards; PMC traffic) and the share of slots
+                the calibration kernel fills.  `roofline_levels` gives the
+                HBM-bound BFS level kernels (scan, expand) at config 2's own
+                level sizes (small: launch-latency bound), and
+                `roofline_levels_full_size` gives scan, expand, make+count and
+                genmoves on the largest level of the tree (config 2 run with
+                the one-ply leaf).
   valu_issue    the leaf's VALU issue rate per SIMD and cycle: its measured
                 instruction count per launch over the live launch time, and
                 its lane instructions per counted position.
