@@ -361,8 +361,9 @@ __global__ __launch_bounds__(BS) void k_b_genmoves(BatchDev *d, int L)
     u16 *out = d->lv[L + 1].mv;
     u64 n = a.n;
     const u32 *incl = a.cnt;
-    __shared__ u16 buf[GEN_LCAP];
+    __shared__ __attribute__((aligned(16))) u16 buf[GEN_LCAP];
     u64 tiles = (n + BS - 1) / BS;
+    const bool aligned = ((uintptr_t)out & 15) == 0;  // arena carvings are 256-byte aligned
     for (u64 t = blockIdx.x; t < tiles; t += gridDim.x) {
         u64 b0 = t * BS, i = b0 + threadIdx.x;
         u64 bEnd = b0 + BS < n ? b0 + BS : n;
@@ -370,16 +371,21 @@ __global__ __launch_bounds__(BS) void k_b_genmoves(BatchDev *d, int L)
         u64 total = (u64)incl[bEnd - 1] - blockStart;
         u64 excl = (i < n && i) ? (u64)incl[i - 1] : 0;
         u64 mine = i < n ? (u64)incl[i] - excl : 0;  // 0 for a table hit: nothing to write
-        if (total <= GEN_LCAP) {
+        const u64 sh = aligned ? (blockStart & 7) : 0;  // the tile's offset (copyMovesOut)
+        if (total + sh <= GEN_LCAP) {
             if (mine) {
-                u16 *dst = buf + (excl - blockStart);
+                u16 *dst = buf + sh + (excl - blockStart);
                 generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) {
                     if (k < mine) dst[k] = m;
                 });
             }
             __syncthreads();
-            u16 *o = out + blockStart;
-            for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
+            if (aligned) {
+                copyMovesOut(buf, out, blockStart, total);
+            } else {
+                u16 *o = out + blockStart;
+                for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
+            }
         } else if (mine) {
             u16 *dst = out + excl;
             generateMoves<C>(a.boards[i], a.states[i], [&](u32 k, u16 m) {

@@ -364,6 +364,27 @@ __device__ __forceinline__ void storeTileStreaming(u32 *__restrict__ tile, const
     }
 }
 
+// Moves of one genmoves block from its LDS tile to out[gs, gs + total).  The
+// tile holds global index g at LDS index g - gs + (gs & 7), so that every
+// 8-move global chunk starts at a multiple of 8 in LDS: whole chunks go out
+// as one 16-byte LDS read and one 16-byte store per lane; the partial chunks
+// at either end move element by element (their other entries belong to the
+// neighbouring blocks).  out must be 16-byte aligned, buf too.
+__device__ __forceinline__ void copyMovesOut(const u16 *__restrict__ buf, u16 *__restrict__ out, u64 gs, u64 total)
+{
+    const u64 ge = gs + total, c0 = gs >> 3, c1 = (ge + 7) >> 3;
+    for (u64 c = c0 + threadIdx.x; c < c1; c += BS) {
+        const u32 l = (u32)((c - c0) << 3);
+        const u64 g = c << 3;
+        if (g >= gs && g + 8 <= ge) {
+            *(uint4 *)(out + g) = *(const uint4 *)(buf + l);
+        } else {
+            for (int j = 0; j < 8; j++)
+                if (g + j >= gs && g + j < ge) out[g + j] = buf[l + j];
+        }
+    }
+}
+
 // first p in [0, n) with incl[p] > c (n if none)
 __device__ __forceinline__ u64 upperBound(const u32 *__restrict__ incl, u64 n, u64 c)
 {

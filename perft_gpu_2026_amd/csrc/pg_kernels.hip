@@ -141,7 +141,7 @@ template <int C>
 __global__ __launch_bounds__(BS) void k_genmoves(const QuadBitBoard *__restrict__ boards, const u8 *__restrict__ states,
                                                  const u32 *__restrict__ incl, u64 n, u16 *__restrict__ out)
 {
-    __shared__ u16 buf[GEN_LCAP];
+    __shared__ __attribute__((aligned(16))) u16 buf[GEN_LCAP];
     u64 b0 = (u64)blockIdx.x * BS;
     u64 i = b0 + threadIdx.x;
     u64 bEnd = b0 + BS < n ? b0 + BS : n;
@@ -152,16 +152,21 @@ __global__ __launch_bounds__(BS) void k_genmoves(const QuadBitBoard *__restrict_
         excl = i ? (u64)incl[i - 1] : 0;
         mine = (u64)incl[i] - excl;  // 0 for a table hit or a duplicate: nothing to write
     }
-    if (total <= GEN_LCAP) {
+    const u64 sh = ((uintptr_t)out & 15) == 0 ? (blockStart & 7) : 0;  // the tile's offset (copyMovesOut)
+    if (total + sh <= GEN_LCAP) {
         if (mine) {
-            u16 *dst = buf + (excl - blockStart);
+            u16 *dst = buf + sh + (excl - blockStart);
             generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) {
                 if (k < mine) dst[k] = m;
             });
         }
         __syncthreads();
-        u16 *o = out + blockStart;
-        for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
+        if (((uintptr_t)out & 15) == 0) {
+            copyMovesOut(buf, out, blockStart, total);
+        } else {
+            u16 *o = out + blockStart;
+            for (u64 j = threadIdx.x; j < total; j += BS) o[j] = buf[j];
+        }
     } else if (mine) {
         u16 *dst = out + excl;
         generateMoves<C>(boards[i], states[i], [&](u32 k, u16 m) {
