@@ -304,13 +304,16 @@ def batch_leg(pg) -> dict:
                            "tuned": tuned_tt}}
 
 
+HOST_TT_MB = 262144  # config 4's 256 GB host tier (its untouched pages cost no memory)
+
+
 def tt_leg(pg) -> dict:
     """Config 4: 64 positions (10 random plies), perft 8 with the transposition
-    tables (64 GB device table + 32 GB host tier), through the launcher (host
+    tables (64 GB device table + 256 GB host tier), through the launcher (host
     recursion above the launch depth, frontier batched on the device)."""
     rows = load_fixture("c4.jsonl")
     ps = [pg.parse_fen(r["fen"]) for r in rows]
-    pg.tt_init(65536, 32768)
+    pg.tt_init(65536, HOST_TT_MB)
     # exactness: depth-5 golden counts (oracle), then TT on == TT off at depth 7
     for r, p in zip(rows, ps):
         assert pg.perft(p, r["golden_depth"], use_tt=True) == r["golden_count"]
@@ -323,12 +326,13 @@ def tt_leg(pg) -> dict:
     counts = [pg.perft(p, 8, use_tt=True) for p in ps]
     dt = time.perf_counter() - t0
     stats = [b - a for a, b in zip(stats0, pg.tt_stats())]
+    assert stats[3] > 0, "the host tier took no probe: it was not allocated"
     # TT off on the first 4 positions at full depth as a spot check
     spot = [pg.perft(p, 8) for p in ps[:4]]
     assert spot == counts[:4], (spot, counts[:4])
     pg.tt_free()
     nodes = sum(counts)
-    return {"workload": "config-4: 64 positions (10 plies, seeds 100-163), perft 8, device TT 64 GB + host TT 32 GB, "
+    return {"workload": "config-4: 64 positions (10 plies, seeds 100-163), perft 8, device TT 64 GB + host TT 256 GB, "
                         "launcher (auto launch depth), one position after another, tables shared",
             "nodes": nodes, "seconds": dt, "nodes_per_s": nodes / dt,
             "device_tt_probes": stats[0], "device_tt_hits": stats[1], "device_tt_stores": stats[2],
@@ -348,7 +352,7 @@ REF_RUNS_TT = [("startpos", 9, "0.174 s on RTX PRO 6000"), ("startpos", 10, "2.0
 def ref_positions_leg(pg, tt_max_depth: int, tt_only: bool = False) -> dict:
     """The reference README's published positions through the launcher, each
     count checked against the published value: without the tables, then
-    startpos with them (64 GB device table, 32 GB host tier, cleared before
+    startpos with them (64 GB device table, 256 GB host tier, cleared before
     each run)."""
     known = {p["name"]: p for p in json.load(open(os.path.join(ROOT, "tests", "golden", "known_positions.json")))["positions"]}
     out = {}
@@ -362,7 +366,7 @@ def ref_positions_leg(pg, tt_max_depth: int, tt_only: bool = False) -> dict:
         dt = time.perf_counter() - t0
         assert c == want, (name, d, c, want)
         out[f"{name}_perft{d}"] = {"nodes": c, "seconds": dt, "nodes_per_s": c / dt, "reference_published": theirs}
-    pg.tt_init(65536, 32768)
+    pg.tt_init(65536, HOST_TT_MB)
     try:
         for name, d, theirs in REF_RUNS_TT:
             if d > tt_max_depth:

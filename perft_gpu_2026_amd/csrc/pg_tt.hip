@@ -163,11 +163,23 @@ int ttClearDevice()
     return PG_OK;
 }
 
+// The host tier is cleared by handing its bucket heads back and taking fresh
+// zeroed ones: a large calloc maps zero pages that the OS fills on first
+// touch, so a 32 GB head array (a 256 GB tier) clears in microseconds instead
+// of a memset of every page, and untouched buckets cost no resident memory.
 int ttClear()
 {
     PG_TRY(ttClearDevice());
     if (g_host.heads) {
-        memset((void *)g_host.heads, 0, (g_host.mask + 1) * sizeof(u32));
+        const u64 buckets = g_host.mask + 1;
+        free(g_host.heads);
+        g_host.heads = (std::atomic<u32> *)calloc(buckets, sizeof(u32));
+        if (!g_host.heads) {
+            hostFree();
+            setError("host TT: its cleared bucket heads could not be allocated");
+            fprintf(stderr, "warning: %s, host TT disabled\n", pg_last_error());  // as hostInit
+            return PG_OK;
+        }
         g_host.used = 1;
     }
     return PG_OK;

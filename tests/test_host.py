@@ -108,6 +108,36 @@ def test_cpu_perft_with_host_tt_matches(engine):
         engine.tt_free()
 
 
+def test_host_tt_clear_takes_fresh_heads(engine):
+    # a clear hands the bucket heads back and takes zeroed ones: nothing is
+    # found right after it, counts stay exact, and a 256 GB tier is taken and
+    # cleared without touching its pages (resident memory stays small)
+    import resource
+    fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
+    p = engine.parse_fen(fen)
+    engine.tt_init(0, 64)
+    try:
+        assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+        assert engine.lib().pg_tt_clear() == 0
+        before = engine.tt_stats()
+        assert engine.perft_cpu(p, 2, use_tt=True) == 2039
+        after = engine.tt_stats()
+        assert after[4] == before[4]  # no host hit: the cleared tier holds nothing
+        assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+    finally:
+        engine.tt_free()
+    rss0 = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss
+    engine.tt_init(0, 262144)
+    try:
+        assert engine.lib().pg_tt_clear() == 0
+        assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
+        st = engine.tt_stats()
+        assert st[5] > 0  # the 256 GB tier was allocated and stored
+        assert resource.getrusage(resource.RUSAGE_SELF).ru_maxrss - rss0 < 4 << 20  # KiB: under 4 GiB touched
+    finally:
+        engine.tt_free()
+
+
 def test_workload_walk_matches_oracle(engine, orc):
     for s in range(200):
         e, pl, d = engine.random_request(s, (6, 40), (3, 6))
