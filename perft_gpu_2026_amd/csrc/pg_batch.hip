@@ -450,8 +450,9 @@ __global__ __launch_bounds__(BS) void k_b_leaf1(BatchDev *d)
 // keeps per-entry partial sums in LDS (atomics) and attributes them per lane.
 // The choice is wave-uniform, so the pair loop does not diverge.  Pairs take
 // 3 bytes of LDS and the kernels 128 VGPRs at most, so four blocks fit a CU
-// and four waves a SIMD, as for k_leaf2q.
-constexpr int BL2_CAP = 2016;
+// and four waves a SIMD, as for k_leaf2q; a window of 2,560 pairs is what
+// four blocks per CU leave beside the per-entry sums.
+constexpr int BL2_CAP = 2560;
 constexpr int BL2_WAVES = 4;
 #define BL2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 4)))
 template <int C>

@@ -207,15 +207,22 @@ __global__ __launch_bounds__(BS) void k_leaf1(const QuadBitBoard *__restrict__ p
 // so the counter's register budget is not shared with the generator's.
 // Pair lists longer than the LDS capacity are processed in windows.  A chunk
 // of 64 entries keeps every lane busy in the child phase (make, count,
-// generate); 64 entries have about 1,750 pairs, so 2,048 pair slots rarely
-// need a second window.  A pair takes 3 bytes of LDS (the move, and the lane
-// of its child in a separate byte array): 32 KB per 4-wave block, so four
-// blocks fit a CU, and the kernels are held to 128 VGPRs, so four waves fit a
-// SIMD (140 VGPRs, 4-byte pairs and three waves per SIMD measured 3.6 %
-// slower; five waves need 96 VGPRs, which spill, and measured 11 % slower).
+// generate).  A pair takes 3 bytes of LDS (the move, and the lane of its
+// child in a separate byte array), and the kernels are held to 128 VGPRs, so
+// four waves fit a SIMD (140 VGPRs, 4-byte pairs and three waves per SIMD
+// measured 3.6 % slower; five waves need 96 VGPRs, which spill, and measured
+// 11 % slower).  The window is as large as four blocks per CU allow: 2,688
+// pairs (40,736 B of LDS per block).  64 entries of the config-2 tree have
+// 1,753 pairs on average; 15 % of the chunks exceeded the 2,016 pairs of the
+// earlier window and took a second one (their straddling lanes generate
+// twice, and each window ends in a partial pass), 0.3 % exceed 2,688: 1.35 %
+// faster.  The table leaf keeps 2,560 (its per-entry sums take 1 KB more),
+// the table-lookup form 2,016 (its LDS bishop table leaves two blocks per CU).
 #define L2_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 4)))
 constexpr int L2_CHUNK = 64;
-constexpr int L2_CAP = 2016;
+constexpr int L2_CAP = 2688;
+constexpr int L2_CAP_TT = 2560;
+constexpr int L2_CAP_MAGIC = 2016;
 constexpr int L2_WAVES = 4;
 
 // Fancy magic slider tables (pg_magic.h) for the table-lookup form of the
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2q_magic(con
                                                                 const u16 *__restrict__ mv, u64 n,
                                                                 u64 *__restrict__ sum, u64 *work)
 {
-    leaf2Body<C, L2_CHUNK, L2_CAP, true, SLM>(pb, ps, pidx, mv, n, sum, work);
+    leaf2Body<C, L2_CHUNK, L2_CAP_MAGIC, true, SLM>(pb, ps, pidx, mv, n, sum, work);
 }
 
 // Two shapes of the same leaf: 64-entry chunks (default) and 32-entry chunks
@@ -480,8 +487,8 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const 
     __shared__ QuadBitBoard sb[L2_WAVES][L2_CHUNK];
     __shared__ u8 ss[L2_WAVES][L2_CHUNK];
     __shared__ u32 sacc[L2_WAVES][L2_CHUNK];
-    __shared__ u16 slm[L2_WAVES][L2_CAP + 1];  // slot L2_CAP: moves outside the window
-    __shared__ u8 sll[L2_WAVES][L2_CAP + 1];
+    __shared__ u16 slm[L2_WAVES][L2_CAP_TT + 1];  // slot L2_CAP_TT: moves outside the window
+    __shared__ u8 sll[L2_WAVES][L2_CAP_TT + 1];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     TTTally tally;
     // chunk k of each wave's sequence is chunk k * P mod nChunks of the level
@@ -514,8 +521,8 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const 
         u32 off = incl - cnt;
         u32 total = __shfl(incl, 63);
         waveLdsSync();
-        for (u32 wb = 0; wb < total; wb += L2_CAP) {
-            if (cnt && off < wb + L2_CAP && off + cnt > wb) {
+        for (u32 wb = 0; wb < total; wb += L2_CAP_TT) {
+            if (cnt && off < wb + L2_CAP_TT && off + cnt > wb) {
                 QuadBitBoard q = sb[w][lane];
                 u8 st = ss[w][lane];
                 u16 *dm = slm[w];
@@ -524,12 +531,12 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const 
                 fr.att = catt;
                 generateMovesF<1 - C>(fr, st, [&](u32 k, u16 m) {
                     u32 j = off + k - wb;
-                    j = j < (u32)L2_CAP ? j : (u32)L2_CAP;
+                    j = j < (u32)L2_CAP_TT ? j : (u32)L2_CAP_TT;
                     dm[j] = m, dl[j] = (u8)lane;
                 });
             }
             waveLdsSync();
-            u32 wn = total - wb < (u32)L2_CAP ? total - wb : (u32)L2_CAP;
+            u32 wn = total - wb < (u32)L2_CAP_TT ? total - wb : (u32)L2_CAP_TT;
             for (u32 j = lane; j < wn; j += 64) {
                 u16 x = slm[w][j];
                 u32 src = sll[w][j];
