@@ -112,7 +112,12 @@ def test_host_tt_clear_takes_fresh_heads(engine):
     # a clear hands the bucket heads back and takes zeroed ones: nothing is
     # found right after it, counts stay exact, and a 256 GB tier is taken and
     # cleared without touching its pages (resident memory stays small)
-    import resource
+    import os
+
+    def resident_bytes():
+        with open("/proc/self/statm") as f:
+            return int(f.read().split()[1]) * os.sysconf("SC_PAGE_SIZE")
+
     fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -"
     p = engine.parse_fen(fen)
     engine.tt_init(0, 64)
@@ -126,14 +131,14 @@ def test_host_tt_clear_takes_fresh_heads(engine):
         assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
     finally:
         engine.tt_free()
-    rss0 = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss
+    rss0 = resident_bytes()
     engine.tt_init(0, 262144)
     try:
         assert engine.lib().pg_tt_clear() == 0
         assert engine.perft_cpu(p, 4, use_tt=True) == 4085603
         st = engine.tt_stats()
         assert st[5] > 0  # the 256 GB tier was allocated and stored
-        assert resource.getrusage(resource.RUSAGE_SELF).ru_maxrss - rss0 < 4 << 20  # KiB: under 4 GiB touched
+        assert resident_bytes() - rss0 < 4 << 30  # under 4 GiB of the tier's pages touched
     finally:
         engine.tt_free()
 
