@@ -2,7 +2,7 @@
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--leaf 1|2]
                   [--no-levels] [--no-batch] [--no-tt] [--no-ref-positions]
-                  [--serve N] [--serve-rate R]
+                  [--serve N] [--serve-rate R] [--copy-probe | --valu-probe | --tt-leg-only]
 
 Workload (one step): perft(7) of the config-2 position -- the start position
 after 8 seeded random plies (seed 2, SplitMix64 walk over canonically sorted
@@ -341,6 +341,24 @@ def tt_leg(pg) -> dict:
             "checked": "depth-5 oracle counts; TT on == TT off at depth 7 (all 64) and depth 8 (first 4)"}
 
 
+def tt_leg_timed(pg) -> dict:
+    """Config 4's timed leg alone, as tt_leg runs it (for counter runs): the
+    tables cleared, then perft 8 of the 64 positions with them, and the table
+    tallies of the leg."""
+    ps = [pg.parse_fen(r["fen"]) for r in load_fixture("c4.jsonl")]
+    pg.tt_init(65536, HOST_TT_MB)
+    pg.lib().pg_tt_clear()
+    stats0 = pg.tt_stats()
+    t0 = time.perf_counter()
+    nodes = sum(pg.perft(p, 8, use_tt=True) for p in ps)
+    dt = time.perf_counter() - t0
+    stats = [b - a for a, b in zip(stats0, pg.tt_stats())]
+    pg.tt_free()
+    return {"workload": "config-4 timed leg", "nodes": nodes, "seconds": dt, "device_tt_probes": stats[0],
+            "device_tt_hits": stats[1], "device_tt_stores": stats[2], "host_tt_probes": stats[3],
+            "host_tt_hits": stats[4], "host_tt_stores": stats[5]}
+
+
 # The reference's own published single-position runs, for context:
 # README.md:51-52 and :58-60 (RTX PRO 6000 Blackwell) and :74 (RTX 4090).
 REF_RUNS = [("startpos", 9, "1.867 s on RTX PRO 6000"), ("kiwipete", 7, "0.177 s on RTX PRO 6000"),
@@ -509,6 +527,8 @@ def main():
                     help="shared transposition table of the serving leg, inside its 200 GB budget")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM calibration copies and exit")
     ap.add_argument("--valu-probe", action="store_true", help="only run the VALU calibration kernel and exit")
+    ap.add_argument("--tt-leg-only", action="store_true",
+                    help="only run config 4's timed leg once (tables cleared first; for counter runs) and exit")
     ap.add_argument("--tt-frontier", type=int, default=-1,
                     help="launcher with the tables: 0 serial recursion, 1 parallel frontier, 2 batched frontier "
                          "(-1: the library default)")
@@ -530,6 +550,10 @@ def main():
     if a.valu_probe:
         pg.init(arena_bytes=1 << 30)
         print(json.dumps(valu_calibration(pg)), flush=True)
+        return
+    if a.tt_leg_only:
+        pg.init(0)
+        print(json.dumps(tt_leg_timed(pg)), flush=True)
         return
     fx = load_fixture("c2.jsonl")[0]
     depth, want = fx["depth"], fx["counts"][-1]
