@@ -954,9 +954,11 @@ static int runGroupSplit(BatchState &B, const GroupPlan &g, const QuadBitBoard *
 // first 1 GiB or 128 MiB (a span that the 256 MB last-level cache can hold
 // makes probes cheaper, and a span too small for the group's stores loses
 // hits), by timing each choice on a sample of the group.  Each choice first
-// runs on one sample (graph capture, first launches, table warm-up) and is
-// timed on a second, disjoint one when the group has room, so a timed run
-// does not hit its own entries.  A span is tried only when it holds at least
+// runs on a quarter of one sample (graph capture and first launches: the
+// graph does not depend on the number of roots, and each trial's table keys
+// are salted, so a larger warm-up buys nothing) and is timed on a second,
+// disjoint sample when the group has room, so a timed run does not hit its
+// own entries.  A span is tried only when it holds at least
 // eight entries per store the whole group is expected to make (the timed
 // sample's stores scaled to the group).  The sample shrinks 4x per ply above
 // depth 4 so that tuning a deep group stays cheap.  Choices are cached per
@@ -981,13 +983,14 @@ static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roo
     for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
     ns = std::min<u64>(roots.size(), ns);
     const u64 t0i = roots.size() >= 2 * ns ? ns : 0;  // timed sample [t0i, t0i + ns)
+    const u64 nw = ns >= 4 ? ns / 4 : 1;               // warm-up sample [0, nw)
     std::vector<u64> tmp(ns);
     u64 stores = 0;  // table stores of the last timed run
     auto trial = [&](GroupPlan g) {
         // a key space of its own: no trial finds what another stored
         g.ttSalt = 0x9E3779B97F4A7C15ULL * ++g_tuneTrials;
         double t = 1e30;
-        if (runGroupSplit(g_batch, g, roots.data(), states.data(), ns, useGraph, tmp.data()) == PG_OK) {
+        if (runGroupSplit(g_batch, g, roots.data(), states.data(), nw, useGraph, tmp.data()) == PG_OK) {
             uint64_t before[6], after[6];
             ttStats(before);
             auto a = std::chrono::steady_clock::now();
