@@ -87,8 +87,9 @@ __device__ __forceinline__ u64 blockSum64(u64 v)
 // (a row of 4,096 at most 256 * 218 each), u64 for group sums (64 tiles each).
 constexpr int TOP_BS = 1024;
 constexpr int TOP_ROWS = 8;
+// Returns the total in every thread.
 template <typename ACC = u32, int ROWS = TOP_ROWS>
-__device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u64 tiles, u64 *__restrict__ tileOffs)
+__device__ __forceinline__ u64 scanTileSums(const u32 *__restrict__ tileSums, u64 tiles, u64 *__restrict__ tileOffs)
 {
     __shared__ u32 wtot[ROWS][TOP_BS / 64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -151,6 +152,15 @@ __device__ __forceinline__ void scanTileSums(const u32 *__restrict__ tileSums, u
         __syncthreads();  // wtot is rewritten by the next pass
     }
     if (tid == 0) tileOffs[tiles] = carry;
+    return carry;
+}
+
+// Publishes a value to a fine-grained pinned host slot that a host thread
+// polls while the stream runs on: one system-scope store, which orders no
+// other write (what later kernels read is ordered by the stream).
+__device__ __forceinline__ void publishToHost(u64 *slot, u64 v)
+{
+    __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ------------------------------------------------- grouped scan (plain BFS)

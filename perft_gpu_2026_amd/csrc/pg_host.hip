@@ -35,6 +35,27 @@ Lib &lib()
     return L;
 }
 
+// Polls a pinned slot until a kernel of stream `st` publishes a value there.
+// After 20 ms it synchronises the stream instead, which reports a failed
+// kernel rather than spinning on a value that never comes.
+int waitPublished(volatile uint64_t *slot, hipStream_t st, uint64_t &v)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 1;; i++) {
+        v = *slot;
+        if (v != HOST_SLOT_EMPTY) return PG_OK;
+        if (!(i & 255) && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
+        __builtin_ia32_pause();
+    }
+    PG_HIP(hipStreamSynchronize(st));
+    v = *slot;
+    if (v == HOST_SLOT_EMPTY) {
+        setError("a kernel did not publish its level total");
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
 int refuseWhileServing(const char *what)
 {
     if (!lib().serving.load()) return PG_OK;
@@ -275,7 +296,7 @@ int initLib(int device, uint64_t arenaBytes)
     arenaBytes &= ~(uint64_t)255;
     PG_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     PG_HIP(hipMalloc(&L.arena, arenaBytes));
-    PG_HIP(hipHostMalloc((void **)&L.hostPinned, PINNED_BYTES, hipHostMallocDefault));
+    PG_HIP(hipHostMalloc((void **)&L.hostPinned, PINNED_BYTES, PINNED_FLAGS));
     // the device address of this GPU's static scan workspace (pg_scan_u32_static)
     PG_TRY(initStaticScratch(device));
     L.arenaBytes = arenaBytes;
@@ -466,7 +487,7 @@ static int frontierPerftParallel(const pg_position *pos, int depth, int ld, u128
         PG_TRY(refuseWhileServing("GPU perft"));
         for (auto &s : g_ttSlots) {
             if (!s.stream) PG_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-            if (!s.pinned) PG_HIP(hipHostMalloc((void **)&s.pinned, PINNED_BYTES, hipHostMallocDefault));
+            if (!s.pinned) PG_HIP(hipHostMalloc((void **)&s.pinned, PINNED_BYTES, PINNED_FLAGS));
         }
         const bool profOn = L.prof.on;
         L.prof.on = false;  // the event profiler is single-threaded

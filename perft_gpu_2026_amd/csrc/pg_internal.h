@@ -112,11 +112,12 @@ struct Profiler {
         pending.push_back({cur, curA, e, b, it});
         cur = -1;
     }
-    // call after the stream has been synchronised
+    // waits for each recorded end event, then adds up the intervals
     void collect()
     {
         for (auto &r : pending) {
             float t = 0;
+            (void)hipEventSynchronize(r.b);  // the caller may not have synchronised the stream
             (void)hipEventElapsedTime(&t, r.a, r.b);
             ms[r.id] += t;
             bytes[r.id] += r.bytes;
@@ -137,10 +138,17 @@ struct Profiler {
 };
 
 // The pinned host buffer of every stream that runs GPU BFS calls (the
-// library's, each serving and frontier slot's): its first bytes take the
-// readbacks, and bytes [PIN_STAGE, PINNED_BYTES) stage a call's inputs for
-// its one host-to-device copy.
-constexpr size_t PINNED_BYTES = 4096, PIN_STAGE = 1024;
+// library's, each serving and frontier slot's), allocated fine-grained
+// (hipHostMallocCoherent) so that a kernel's system-scope store reaches the
+// host while the stream still runs: its first bytes take the readbacks, u64
+// slot PIN_POLL takes the level totals the scan publishes for the host to
+// poll, and bytes [PIN_STAGE, PINNED_BYTES) stage a call's inputs for its one
+// host-to-device copy.
+constexpr size_t PINNED_BYTES = 4096, PIN_STAGE = 1024, PIN_POLL = 8;
+constexpr unsigned PINNED_FLAGS = hipHostMallocCoherent;
+// what a polled slot holds until a kernel publishes (no level total reaches it)
+constexpr uint64_t HOST_SLOT_EMPTY = ~0ull;
+int waitPublished(volatile uint64_t *slot, hipStream_t st, uint64_t &v);
 
 struct Lib {
     Profiler prof;

@@ -70,9 +70,11 @@ __global__ __launch_bounds__(BS) void k_scan_groups(const u32 *__restrict__ tile
 }
 
 // Levels of at most 64 groups (4,096 tiles, 1,048,576 entries): the group
-// pass and the scan of the group sums in one block, one launch fewer
+// pass and the scan of the group sums in one block, one launch fewer.  With
+// `hostTotal` non-null the total is also published to that pinned host slot.
 __global__ __launch_bounds__(TOP_BS) void k_scan_small(const u32 *__restrict__ tileSums, u64 tiles,
-                                                       u32 *__restrict__ tileIn, u64 *__restrict__ groupOffs)
+                                                       u32 *__restrict__ tileIn, u64 *__restrict__ groupOffs,
+                                                       u64 *hostTotal)
 {
     __shared__ u32 gs[SCAN_GROUP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -84,15 +86,20 @@ __global__ __launch_bounds__(TOP_BS) void k_scan_small(const u32 *__restrict__ t
     if (w == 0) {
         const u32 v = gs[lane], inc = waveInclScan(v);  // at most 64 groups of 64 * 256 * 218
         if ((u64)lane < G) groupOffs[lane] = inc - v;
-        if (lane == 63) groupOffs[G] = inc;
+        if (lane == 63) {
+            groupOffs[G] = inc;
+            if (hostTotal) publishToHost(hostTotal, inc);
+        }
     }
 }
 
-// exclusive scan of the G group sums into groupOffs[0..G] (one block)
+// exclusive scan of the G group sums into groupOffs[0..G] (one block), the
+// total also to `hostTotal` when non-null
 __global__ __launch_bounds__(TOP_BS) void k_scan_top(const u32 *__restrict__ groupSums, u64 groups,
-                                                     u64 *__restrict__ groupOffs)
+                                                     u64 *__restrict__ groupOffs, u64 *hostTotal)
 {
-    scanTileSums<u64, 2>(groupSums, groups, groupOffs);
+    const u64 total = scanTileSums<u64, 2>(groupSums, groups, groupOffs);
+    if (hostTotal && threadIdx.x == 0) publishToHost(hostTotal, total);
 }
 
 // Inclusive counts from the group and in-group offsets: one wave per
@@ -591,6 +598,13 @@ __global__ __launch_bounds__(BS) void k_upsweep(const QuadBitBoard *__restrict__
     if (tt) ttFlush(ctr, tally);
 }
 
+// one u64 of device memory to a pinned host slot (the count at the end of a
+// call, for the host to poll instead of a readback copy and a synchronise)
+__global__ void k_publish_u64(const u64 *__restrict__ src, u64 *slot)
+{
+    if (threadIdx.x == 0) publishToHost(slot, *src);
+}
+
 __global__ __launch_bounds__(BS) void k_sum_u64(const u64 *__restrict__ v, u64 n, u64 *__restrict__ sum)
 {
     u64 i = (u64)blockIdx.x * BS + threadIdx.x;
@@ -660,16 +674,20 @@ static bool allocScanBufs(Bump &A, u64 n, ScanBufs &b)
     b.groupSums = A.alloc<u32>(G);
     return b.groupOffs && b.tileIn && b.groupSums;
 }
-static int launchScan(u32 *d, u64 n, const u32 *tileSums, const ScanBufs &b, hipStream_t st, u32 *parts = nullptr)
+// With `hostTotal` (a slot of a fine-grained pinned buffer) the total is
+// published there as soon as it is known, before the apply pass runs.
+static int launchScan(u32 *d, u64 n, const u32 *tileSums, const ScanBufs &b, hipStream_t st, u32 *parts = nullptr,
+                      u64 *hostTotal = nullptr)
 {
     if (!n) return PG_OK;
     const u64 T = scanTiles(n), G = scanGroupCount(T);
     if (G <= (u64)SCAN_GROUP) {
-        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(TOP_BS), 0, st, tileSums, T, b.tileIn, b.groupOffs);
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(TOP_BS), 0, st, tileSums, T, b.tileIn, b.groupOffs,
+                           hostTotal);
     } else {
         hipLaunchKernelGGL(k_scan_groups, dim3((unsigned)((G + 15) / 16)), dim3(BS), 0, st, tileSums, T, b.tileIn,
                            b.groupSums);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_BS), 0, st, b.groupSums, G, b.groupOffs);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(TOP_BS), 0, st, b.groupSums, G, b.groupOffs, hostTotal);
     }
     hipLaunchKernelGGL(k_scan_apply, dim3(gridFor(n, 4 * BS)), dim3(BS), 0, st, d, n, b.tileIn, b.groupOffs, parts,
                        parts ? expandPartsCap(n) : 0ull);
@@ -866,10 +884,11 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
                                tileSums);
         }
         PG_HIP(hipGetLastError());
-        PG_TRY(launchScan(cnt, n, tileSums, sb, st, parts));
-        PG_HIP(hipMemcpyAsync(pinned, sb.groupOffs + scanTotalIndex(n), sizeof(u64), hipMemcpyDeviceToHost, st));
-        PG_HIP(hipStreamSynchronize(st));
-        u64 c = pinned[0];
+        volatile u64 *slot = pinned + PIN_POLL;  // the level total, published by the scan
+        *slot = HOST_SLOT_EMPTY;
+        PG_TRY(launchScan(cnt, n, tileSums, sb, st, parts, pinned + PIN_POLL));
+        u64 c;
+        PG_TRY(waitPublished(slot, st, c));
         lv.push_back({ob, os, cnt, known, redir, val, n});
         if (c == 0) {
             // every entry was a table hit, a duplicate or terminal: nothing below
@@ -935,11 +954,14 @@ static int bfsPerftTT(const QuadBitBoard &root, u8 state, int color, int depth, 
 
 // --------------------------------------------------------------- BFS driver
 // perft_gpu_host_bfs (perft_kernels.cu:620-990): root moves on the host, then
-// levels 1 .. depth-leafPlies-1 on the device with one 8-byte readback of the
-// next level's size per level, then the leaf.  Returns PG_ERR_ARENA_OOM (and
-// no count) when a level does not fit; the caller recomputes one ply down.
-int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, bool useTT, void *arena,
-             size_t arenaBytes, hipStream_t st, u64 *pinned, uint64_t *count, size_t *peakBytes)
+// levels 1 .. depth-leafPlies-1 on the device, the host learning each next
+// level's size from the total the scan publishes to a pinned slot (the
+// reference reads it back with a copy), then the leaf.  Returns
+// PG_ERR_ARENA_OOM (and no count) when a level does not fit; the caller
+// recomputes one ply down.
+static int bfsPerftLevels(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, bool useTT,
+                          void *arena, size_t arenaBytes, hipStream_t st, u64 *pinned, uint64_t *count,
+                          size_t *peakBytes)
 {
     if (depth <= 0) {
         *count = 1;
@@ -994,13 +1016,17 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
         P.begin(K_MAKECOUNT, st);
         PG_TRY(launchMakecount(pb, ps, idx, mv, pc, n, ob, os, cnt, tileSums, st));
         P.end(43.0 * n + 33.0 * nParents, (double)n, st);
+        // the scan publishes the level total to the pinned slot, which the
+        // host polls (no readback copy, no stream synchronisation)
+        volatile u64 *slot = pinned + PIN_POLL;
+        *slot = HOST_SLOT_EMPTY;
         P.begin(K_SCAN, st);
-        PG_TRY(launchScan(cnt, n, tileSums, sb, st, parts));
+        PG_TRY(launchScan(cnt, n, tileSums, sb, st, parts, pinned + PIN_POLL));
         P.end(8.0 * n, (double)n, st);
-        PG_HIP(hipMemcpyAsync(pinned, sb.groupOffs + scanTotalIndex(n), sizeof(u64), hipMemcpyDeviceToHost, st));
-        PG_HIP(hipStreamSynchronize(st));
-        u64 c = pinned[0];
+        u64 c;
+        PG_TRY(waitPublished(slot, st, c));
         if (c == 0) {
+            PG_HIP(hipStreamSynchronize(st));  // the apply pass may still run
             *count = 0;
             if (peakBytes) *peakBytes = A.peak;
             P.collect();
@@ -1029,12 +1055,29 @@ int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafP
     P.begin(leafPlies == 1 ? K_LEAF1 : K_LEAF2, st);
     PG_TRY(launchLeaf(pb, ps, idx, mv, pc, n, leafPlies, dSum, st, dSum + 1));
     P.end(6.0 * n + 33.0 * nParents, (double)n, st);
-    PG_HIP(hipMemcpyAsync(pinned, dSum, sizeof(u64), hipMemcpyDeviceToHost, st));
-    PG_HIP(hipStreamSynchronize(st));
+    volatile u64 *slot = pinned + PIN_POLL;
+    *slot = HOST_SLOT_EMPTY;
+    hipLaunchKernelGGL(k_publish_u64, dim3(1), dim3(64), 0, st, dSum, pinned + PIN_POLL);
+    PG_HIP(hipGetLastError());
+    u64 total;
+    PG_TRY(waitPublished(slot, st, total));
     P.collect();
-    *count = pinned[0];
+    *count = total;
     if (peakBytes) *peakBytes = A.peak;
     return PG_OK;
+}
+
+// A call that fails (or stops at a level with no children) may leave kernels
+// queued on its buffer, which the caller may free or reuse next: such a
+// return drains the stream first.  A complete call ends with the host having
+// seen the last kernel's count, so nothing of it is left to run.
+int bfsPerft(const QuadBitBoard &root, u8 state, int color, int depth, int leafPlies, bool useTT, void *arena,
+             size_t arenaBytes, hipStream_t st, u64 *pinned, uint64_t *count, size_t *peakBytes)
+{
+    int s = bfsPerftLevels(root, state, color, depth, leafPlies, useTT, arena, arenaBytes, st, pinned, count,
+                           peakBytes);
+    if (s != PG_OK) (void)hipStreamSynchronize(st);
+    return s;
 }
 
 }  // namespace pg

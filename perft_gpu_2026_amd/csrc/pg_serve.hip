@@ -208,7 +208,7 @@ static int serveCreateLocked(const pg_serve_config *cfg, void **handle)
         s.arena = (char *)L.arena + k * per;
         s.bytes = per;
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipHostMalloc((void **)&s.pinned, PINNED_BYTES, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&s.pinned, PINNED_BYTES, PINNED_FLAGS) != hipSuccess) {
             setError("serving slot %d: stream/pinned allocation failed", k);
             delete S;
             return PG_ERR_HIP;
