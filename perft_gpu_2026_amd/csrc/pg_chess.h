@@ -148,7 +148,13 @@ PG_HD bool meets(BB a, BB b)
 PG_HD BB rev64(BB x) { return BB(__builtin_bitreverse32(x.hi), __builtin_bitreverse32(x.lo), 0); }
 
 // Board shift by S (positive: towards h8).  On the device one 64-bit shift
-// instruction; a shift by 32 is a move of halves.
+// instruction; a shift by 32 is a move of halves.  (The compiler would split
+// it into a 32-bit shift and an align per half, 10 % more VALU instructions
+// in the fused leaf.)  Each shift is followed by `s_nop 1`: the wave waits two
+// cycles before its next instruction, which mostly reads the shift's result,
+// and the other waves of the SIMD issue meanwhile.  On the config-2 leaf that
+// took 5.91 to 5.81 ms with the same VALU instructions; `s_nop 0` measured the
+// same, `s_nop 3` 5.95 ms and `s_nop 7` 6.36 ms.
 template <int S>
 PG_HD constexpr u64 shift(u64 b)
 {
@@ -166,8 +172,8 @@ PG_HD BB shift(BB b)
         return BB(b.hi, 0u, 0);
     } else {
         u64 r;
-        if constexpr (S > 0) asm("v_lshlrev_b64 %0, %2, %1" : "=v"(r) : "v"(b.v()), "n"(S));
-        else asm("v_lshrrev_b64 %0, %2, %1" : "=v"(r) : "v"(b.v()), "n"(-S));
+        if constexpr (S > 0) asm("v_lshlrev_b64 %0, %2, %1\n\ts_nop 1" : "=v"(r) : "v"(b.v()), "n"(S));
+        else asm("v_lshrrev_b64 %0, %2, %1\n\ts_nop 1" : "=v"(r) : "v"(b.v()), "n"(-S));
         return BB(r);
     }
 #else
