@@ -134,7 +134,7 @@ def cpu_configs(pg, threads: int = 16, c3_sample: int = 128) -> dict:
 # kernel (FETCH_SIZE reads half the bytes: x2.0; WRITE_SIZE exact: x1.0);
 # the writes are mostly the chunk queue's atomics.
 # Commands and figures: profiles/leaf2_config2_pmc.md.
-PMC_TRAFFIC = {"leaf2": 66168.3 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
+PMC_TRAFFIC = {"leaf2": 66200.6 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
 
 # VALU work per launch of the same leaf: SQ_INSTS_VALU (wave64 instructions)
 # and SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64), the share of lanes
@@ -142,7 +142,7 @@ PMC_TRAFFIC = {"leaf2": 66168.3 * 1024 * 2.0 + 7650.3 * 1024 * 1.0}
 # (profiles/leaf2_config2_pmc.md).  Divided by the live launch time, the
 # instruction count gives the issue rate per SIMD and cycle (1,024 SIMDs at
 # the 2.4 GHz peak clock).
-PMC_VALU = {"leaf2": (4.2418e9, 0.914, 0.1958)}  # + the share of issue slots that issued two (VALU2 / VALU)
+PMC_VALU = {"leaf2": (4.2418e9, 0.914, 0.2193)}  # + the share of issue slots that issued two (VALU2 / VALU)
 SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
