@@ -13,6 +13,8 @@
 //   k_expand      parent index of every child: first-child scatter + max-scan
 //                 in LDS, coalesced non-temporal 16-byte stores    (HBM)
 //   k_genmoves    moves of every board, staged in LDS, written coalesced
+//   k_level_small all of the above in one block, for levels of at most
+//                 SMALL_LEVEL entries (the first levels of every call)
 // and after the last level a leaf kernel sums perft(1) or perft(2) of every
 // entry (k_leaf; the 2-ply form is the reference's fused leaf, here with the
 // child moves consumed as they are generated -- no per-thread move list).
@@ -142,6 +144,54 @@ __global__ __launch_bounds__(BS) void k_expand(const u32 *__restrict__ incl, u64
     u32 o[EXPAND_PER];
     expandTileParents(incl, parts[blockIdx.x], parts[blockIdx.x + 1], c0, tile, wmax, o);
     storeTileStreaming(tile, o, parentIdx, c0, c);
+}
+
+// A whole small level in one block of SMALL_LEVEL threads, one entry each:
+// make + exact count, a block scan of the counts, the total published to the
+// pinned host slot, then every child written as (parent index, move) at its
+// offset.  The child arrays are sized for n * MAX_MOVES before the launch, so
+// nothing waits for the total; this replaces five launches (make+count, two
+// scan passes, expand, genmoves) where each would run a few microseconds.
+constexpr u32 SMALL_LEVEL = 1024;
+template <int C>  // side to move in the parent boards
+__global__ __launch_bounds__(SMALL_LEVEL) void k_level_small(const QuadBitBoard *__restrict__ pb,
+                                                             const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                             const u16 *__restrict__ mv, u32 n,
+                                                             QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
+                                                             u32 *__restrict__ nidx, u16 *__restrict__ nmv,
+                                                             u64 *hostTotal)
+{
+    __shared__ u32 wsum[SMALL_LEVEL / 64];
+    const u32 i = threadIdx.x, lane = i & 63, w = i >> 6;
+    QuadBitBoard q{};
+    u8 st = 0;
+    u32 cnt = 0;
+    if (i < n) {
+        const u32 p = pidx[i];
+        q = pb[p];
+        st = ps[p];
+        makeMove<C>(q, st, mv[i]);
+        ob[i] = q;
+        os[i] = st;
+        cnt = countChildren<1 - C>(q, st);
+    }
+    const u32 incl = waveInclScan(cnt);  // at most 1024 * 218 children
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    u32 before = 0, total = 0;
+#pragma unroll
+    for (u32 k = 0; k < SMALL_LEVEL / 64; k++) {
+        const u32 t = wsum[k];
+        before += k < w ? t : 0u;
+        total += t;
+    }
+    if (i == 0) publishToHost(hostTotal, total);
+    if (cnt) {
+        const u32 off = before + incl - cnt;
+        generateMoves<1 - C>(q, st, [&](u32 k, u16 m) {
+            if (k < cnt) nmv[off + k] = m, nidx[off + k] = i;
+        });
+    }
 }
 
 template <int C>
@@ -1003,6 +1053,40 @@ static int bfsPerftLevels(const QuadBitBoard &root, u8 state, int color, int dep
     int last = depth - leafPlies;  // level whose entries feed the leaf
     for (int lvl = 1; lvl < last; lvl++) {
         int pc = color ^ ((lvl - 1) & 1);  // side to move in the parent boards
+        if (n <= SMALL_LEVEL) {
+            // one block does the whole level; the children's arrays take
+            // the bound n * MAX_MOVES (at most 1.5 MB)
+            QuadBitBoard *ob = A.alloc<QuadBitBoard>(n);
+            u8 *os = A.alloc<u8>(n);
+            u32 *nidx = A.alloc<u32>(n * MAX_MOVES);
+            u16 *nmv = A.alloc<u16>(n * MAX_MOVES);
+            if (!ob || !os || !nidx || !nmv) return PG_ERR_ARENA_OOM;
+            volatile u64 *slot = pinned + PIN_POLL;
+            *slot = HOST_SLOT_EMPTY;
+            if (pc == WHITE)
+                hipLaunchKernelGGL(k_level_small<WHITE>, dim3(1), dim3(SMALL_LEVEL), 0, st, pb, ps, idx, mv, (u32)n,
+                                   ob, os, nidx, nmv, pinned + PIN_POLL);
+            else
+                hipLaunchKernelGGL(k_level_small<BLACK>, dim3(1), dim3(SMALL_LEVEL), 0, st, pb, ps, idx, mv, (u32)n,
+                                   ob, os, nidx, nmv, pinned + PIN_POLL);
+            PG_HIP(hipGetLastError());
+            u64 c;
+            PG_TRY(waitPublished(slot, st, c));
+            if (c == 0) {
+                PG_HIP(hipStreamSynchronize(st));  // the block may still be running
+                *count = 0;
+                if (peakBytes) *peakBytes = A.peak;
+                P.collect();
+                return PG_OK;
+            }
+            pb = ob;
+            ps = os;
+            idx = nidx;
+            mv = nmv;
+            nParents = n;
+            n = c;
+            continue;
+        }
         u64 tiles = (n + BS - 1) / BS;
         QuadBitBoard *ob = A.alloc<QuadBitBoard>(n);
         u8 *os = A.alloc<u8>(n);
