@@ -289,6 +289,29 @@ def test_expand_matches_searchsorted(gpu, n, zero_frac):
     assert np.array_equal(out.to_numpy(np.uint32, c), want)
 
 
+def test_small_level_boundary_against_oracle(gpu, orc):
+    # A level of at most 1,024 entries runs in one block (k_level_small),
+    # a larger one through make+count, scan, expand and genmoves.  The second
+    # level holds perft(2) entries: positions with exactly 1,024 and 1,025,
+    # and a few on either side, at perft 4 and 5 against the oracle.
+    want = {1024: 2, 1025: 2, "below": 3, "above": 3}
+    picked = []
+    for seed in range(20_000):
+        if not any(want.values()):
+            break
+        p = orc.random_position(seed, 4 + seed % 40, require_nonterminal=True)
+        c2 = orc.perft(p, 2)
+        key = c2 if c2 in (1024, 1025) else ("below" if 900 <= c2 < 1024 else "above" if 1025 < c2 <= 1200 else None)
+        if key is not None and want.get(key, 0) > 0:
+            want[key] -= 1
+            picked.append((orc.write_fen(p), c2))
+    assert not any(want.values()), want
+    for fen, c2 in picked:
+        for d in (4, 5):
+            print(f"[gpu] small-level boundary perft2={c2} d={d} {fen}", flush=True)
+            assert gpu.perft_bfs(gpu.parse_fen(fen), d) == orc.perft(orc.parse_fen(fen), d, threads=8), (fen, d)
+
+
 def test_makecount_and_genmoves_match_host(gpu):
     # one colour per call: white-to-move parents
     roots = [p for p in (gpu.random_position(s, 2 * (s % 30)) for s in range(400)) if p.color == 0]
