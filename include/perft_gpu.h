@@ -112,7 +112,10 @@ int pg_random_request(uint64_t seed, int pliesLo, int pliesHi, int depthLo, int 
  * library arena, or [gpuBuffer, gpuBuffer+bufferBytes) when gpuBuffer != NULL
  * (a device pointer; lets a caller force OOM with a tiny buffer, as the
  * reference's bufferSize parameter does).  Returns PG_ERR_ARENA_OOM and no count
- * when a level does not fit.  useTT != 0 probes/stores the device TT. */
+ * when a level does not fit.  useTT != 0 probes/stores the device TT.  It
+ * returns only after the device is done with the buffer: the host learns each
+ * level's size and the count from values the kernels publish to pinned host
+ * memory, and a call that fails or ends early drains its stream. */
 int pg_perft_bfs(const pg_position *pos, int depth, int useTT,
                  void *gpuBuffer, uint64_t bufferBytes, uint64_t *count);
 
