@@ -601,6 +601,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
         u64 e = base + lane, hv = 0;
         bool valid = lane < B_CHUNK && e < n, hit = false;
         BB catt;  // the child's enemy attacks, from its count, for its generation
+        Hash128 h{};  // for the probe and, on a miss, the store
         if (valid) {
             u32 pi = a.idx[e];
             QuadBitBoard q = p.boards[pi];
@@ -608,7 +609,8 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
             makeMove<C>(q, st, a.mv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            hit = ttProbe(tt, ttMask, salted(hashPosition(q, st, 1 - C), salt), 2, &hv);
+            h = salted(hashPosition(q, st, 1 - C), salt);
+            hit = ttProbe(tt, ttMask, h, 2, &hv);
             if (!hit) {
                 Frame fr;
                 cnt = countChildrenFrame<1 - C>(q, st, FillSliders(), fr);
@@ -650,7 +652,7 @@ __global__ __launch_bounds__(BL2_WAVES * 64) BL2_OCCUPANCY void k_b_leaf2v(Batch
         if (valid) {
             u64 v = hit ? hv : sacc[w][lane];
             a.val[e] = v;
-            if (store) ttStore(tt, ttMask, salted(hashPosition(sb[w][lane], ss[w][lane], 1 - C), salt), 2, v);
+            if (store) ttStore(tt, ttMask, h, 2, v);
         }
         tally.add(valid, hit, store);
         waveLdsSync();

@@ -559,6 +559,7 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const 
         u64 e = base + lane, hv = 0;
         bool valid = lane < L2_CHUNK && e < n, hit = false;
         BB catt;  // the child's enemy attacks, from its count, for its generation
+        Hash128 h{};  // for the probe and, on a miss, the store
         if (valid) {
             u32 p = pidx[e];
             QuadBitBoard q = pb[p];
@@ -566,7 +567,8 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const 
             makeMove<C>(q, st, mv[e]);
             sb[w][lane] = q;
             ss[w][lane] = st;
-            hit = tt && ttProbe(tt, mask, hashPosition(q, st, 1 - C), 2, &hv);
+            h = hashPosition(q, st, 1 - C);
+            hit = tt && ttProbe(tt, mask, h, 2, &hv);
             if (!hit) {
                 Frame fr;
                 cnt = countChildrenFrame<1 - C>(q, st, FillSliders(), fr);
@@ -609,7 +611,7 @@ __global__ __launch_bounds__(L2_WAVES * 64) L2_OCCUPANCY void k_leaf2_val(const 
             u64 v = hit ? hv : sacc[w][lane];
             val[e] = v;
             if (tt && !hit) {
-                ttStore(tt, mask, hashPosition(sb[w][lane], ss[w][lane], 1 - C), 2, v);
+                ttStore(tt, mask, h, 2, v);
                 store = true;
             }
         }
