@@ -705,8 +705,6 @@ static int launchMakecount(const QuadBitBoard *pb, const u8 *ps, const u32 *pidx
 }
 
 // scan: tileSums (per 256) -> group offsets -> in-place inclusive scan.
-// offs needs scanTotalIndex(n) + 1 u64 (at most ceil(n/256) + 1); the total
-// lands in offs[scanTotalIndex(n)].
 // Scan buffers of an n-entry level: group offsets (G + 1 u64, the total at
 // [G]), in-group tile offsets (T u32) and group sums (G u32), T = ceil(n/256),
 // G = ceil(T/64).  With `parts` (capacity expandPartsCap(n)) the scan also
@@ -716,7 +714,6 @@ struct ScanBufs {
     u32 *tileIn, *groupSums;
 };
 static u64 scanTiles(u64 n) { return (n + BS - 1) / BS; }
-static u64 scanTotalIndex(u64 n) { return scanGroupCount(scanTiles(n)); }
 static u64 expandPartsCap(u64 n) { return (n * MAX_MOVES + EXPAND_TILE - 1) / EXPAND_TILE + 1; }
 static bool allocScanBufs(Bump &A, u64 n, ScanBufs &b)
 {
