@@ -966,6 +966,16 @@ static int runGroupSplit(BatchState &B, const GroupPlan &g, const QuadBitBoard *
 // depth 4 so that tuning a deep group stays cheap.  Choices are cached per
 // (depth, log2 size, table requested).
 constexpr int TUNE_MAX_DEPTH = 7;  // groups this deep are not tuned
+// Deeper groups than this are leaf-bound: the fused two-ply leaf won every
+// trial there (config 3 with and without the table), so the one-ply leaf is
+// not tried on them, and without the table they get the fixed plan at once,
+// since the grid made no difference either (config 3's tuned pass took
+// 0.5466 / 0.5457 / 0.5457 s at 4 / 8 / 16 blocks per CU).  With the table
+// their grid and table span are still tried (about 3 % on config 3's table
+// pass).  The one-ply trials at depths 5 and 6 were most of the first call:
+// config 3's first call took 0.95 s with them and 0.57 s without, the tuned
+// pass 0.547 s either way.
+constexpr int TUNE_ONE_PLY_MAX_DEPTH = 4;
 static const int TUNE_GRIDS[] = {4, 8, 16};
 static const int TUNE_SPANS[] = {26, 23};  // log2 entries: 1 GiB and 128 MiB of 16-B entries
 static u64 g_tuneTrials = 0;               // trials run so far (their key salts)
@@ -980,7 +990,8 @@ static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roo
     // Deep groups are leaf-bound, where the fused two-ply leaf always wins
     // and the table always pays; a one-ply trial there would cost more than
     // a whole shallow batch (startpos children at depth 9: minutes).
-    if (depth >= TUNE_MAX_DEPTH) return g_batch.tuned[key] = {2, useTT, forcedGrid ? forcedGrid : 8, forcedSpan};
+    if (depth >= TUNE_MAX_DEPTH || (!useTT && depth > TUNE_ONE_PLY_MAX_DEPTH))
+        return g_batch.tuned[key] = {2, useTT, forcedGrid ? forcedGrid : 8, forcedSpan};
     u64 ns = 256;
     for (int d = depth; d > 4 && ns > 8; d--) ns >>= 2;
     ns = std::min<u64>(roots.size(), ns);
@@ -1008,7 +1019,7 @@ static TunedPlan chooseGroupPlan(int depth, const std::vector<QuadBitBoard> &roo
     TunedPlan choice{2, useTT, grid0, forcedSpan};
     u64 choiceStores = 0;
     for (int tt = useTT ? 1 : 0; tt >= 0; tt--) {
-        for (int p = 1; p <= 2 && p < depth; p++) {
+        for (int p = depth > TUNE_ONE_PLY_MAX_DEPTH ? 2 : 1; p <= 2 && p < depth; p++) {
             double t = trial(GroupPlan{depth, p, depth - p, tt != 0, grid0, forcedSpan});
             if (t < best) best = t, choice = {p, tt != 0, grid0, forcedSpan}, choiceStores = stores;
         }
