@@ -18,6 +18,8 @@
 // and after the last level a leaf kernel sums perft(1) or perft(2) of every
 // entry (k_leaf; the 2-ply form is the reference's fused leaf, here with the
 // child moves consumed as they are generated -- no per-thread move list).
+// The host learns each level's size, and at the end the count, from values
+// the scan (and k_publish_u64) store into a pinned host slot that it polls.
 //
 // Indices are u32 (a level holds < 2^32 entries; larger levels report
 // PG_ERR_ARENA_OOM and the caller splits one ply down), sizes are 64-bit.
