@@ -87,7 +87,7 @@ static std::atomic<uint64_t> g_oomCount{0};
 static size_t g_lastPeak = 0;
 static size_t g_peakSinceReset = 0;  // largest g_lastPeak since resetPeakSinceReset
 static uint64_t g_arenaLimit = 0;  // 0 = whole arena; a test hook to force OOM
-int g_leafPlies = 2;  // fused two-ply leaf (measured 2x the one-ply leaf on config 2)
+int g_leafPlies = 2;  // fused two-ply leaf (config 2 perft 7: 5.93 ms, 6.95 ms with the one-ply leaf)
 static bool g_frontierBatch = true;  // launcher: batch the frontier above the launch depth
 // launcher with the tables: the reference's serial recursion, the unique
 // frontier over parallel host threads, or the unique frontier through the
