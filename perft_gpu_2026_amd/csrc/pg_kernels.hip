@@ -17,6 +17,8 @@
 //                 most FUSE_MAX_CHILDREN children
 //   k_level_small all of the above in one block, for levels of at most
 //                 SMALL_LEVEL entries (the first levels of every call)
+//   k_level_root  the same for the first level, from the root and its moves
+//                 passed as kernel arguments
 // and after the last level a leaf kernel sums perft(1) or perft(2) of every
 // entry (k_leaf; the 2-ply form is the reference's fused leaf, here with the
 // child moves consumed as they are generated -- no per-thread move list).
@@ -162,24 +164,17 @@ __global__ __launch_bounds__(BS) void k_expand(const u32 *__restrict__ incl, u64
 // nothing waits for the total; this replaces five launches (make+count, two
 // scan passes, expand, genmoves) where each would run a few microseconds.
 constexpr u32 SMALL_LEVEL = 1024;
+// the block's part after the make: child i (valid when i < n) is written,
+// counted, scanned, published and expanded
 template <int C>  // side to move in the parent boards
-__global__ __launch_bounds__(SMALL_LEVEL) void k_level_small(const QuadBitBoard *__restrict__ pb,
-                                                             const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
-                                                             const u16 *__restrict__ mv, u32 n,
-                                                             QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
-                                                             u32 *__restrict__ nidx, u16 *__restrict__ nmv,
-                                                             u64 *hostTotal)
+__device__ __forceinline__ void levelSmallRest(const QuadBitBoard &q, u8 st, bool valid, QuadBitBoard *__restrict__ ob,
+                                               u8 *__restrict__ os, u32 *__restrict__ nidx, u16 *__restrict__ nmv,
+                                               u64 *hostTotal)
 {
     __shared__ u32 wsum[SMALL_LEVEL / 64];
     const u32 i = threadIdx.x, lane = i & 63, w = i >> 6;
-    QuadBitBoard q{};
-    u8 st = 0;
     u32 cnt = 0;
-    if (i < n) {
-        const u32 p = pidx[i];
-        q = pb[p];
-        st = ps[p];
-        makeMove<C>(q, st, mv[i]);
+    if (valid) {
         ob[i] = q;
         os[i] = st;
         cnt = countChildren<1 - C>(q, st);
@@ -201,6 +196,49 @@ __global__ __launch_bounds__(SMALL_LEVEL) void k_level_small(const QuadBitBoard 
             if (k < cnt) nmv[off + k] = m, nidx[off + k] = i;
         });
     }
+}
+
+template <int C>
+__global__ __launch_bounds__(SMALL_LEVEL) void k_level_small(const QuadBitBoard *__restrict__ pb,
+                                                             const u8 *__restrict__ ps, const u32 *__restrict__ pidx,
+                                                             const u16 *__restrict__ mv, u32 n,
+                                                             QuadBitBoard *__restrict__ ob, u8 *__restrict__ os,
+                                                             u32 *__restrict__ nidx, u16 *__restrict__ nmv,
+                                                             u64 *hostTotal)
+{
+    const u32 i = threadIdx.x;
+    QuadBitBoard q{};
+    u8 st = 0;
+    if (i < n) {
+        const u32 p = pidx[i];
+        q = pb[p];
+        st = ps[p];
+        makeMove<C>(q, st, mv[i]);
+    }
+    levelSmallRest<C>(q, st, i < n, ob, os, nidx, nmv, hostTotal);
+}
+
+// The first level of a call, from the root and its moves passed as kernel
+// arguments (no host-to-device copy before the first kernel); it also clears
+// the call's two device counters (the leaf's sum and chunk queue).
+struct RootArgs {
+    QuadBitBoard board;
+    u32 n;
+    u8 state;
+    u16 moves[MAX_MOVES];
+};
+template <int C>
+__global__ __launch_bounds__(SMALL_LEVEL) void k_level_root(const RootArgs r, QuadBitBoard *__restrict__ ob,
+                                                            u8 *__restrict__ os, u32 *__restrict__ nidx,
+                                                            u16 *__restrict__ nmv, u64 *hostTotal,
+                                                            u64 *__restrict__ counters)
+{
+    const u32 i = threadIdx.x;
+    if (i < 2) counters[i] = 0;
+    QuadBitBoard q = r.board;
+    u8 st = r.state;
+    if (i < r.n) makeMove<C>(q, st, r.moves[i]);
+    levelSmallRest<C>(q, st, i < r.n, ob, os, nidx, nmv, hostTotal);
 }
 
 template <int C>
@@ -1101,21 +1139,64 @@ static int bfsPerftLevels(const QuadBitBoard &root, u8 state, int color, int dep
         if (peakBytes) *peakBytes = A.peak;
         return s;
     }
-    RootBlock rb;
-    PG_TRY(uploadRoot(A, root, state, rootMoves, n1, 2, pinned, st, rb));  // [leaf sum, leaf chunk queue]
-    const QuadBitBoard *b0 = rb.b0;
-    const u8 *s0 = rb.s0;
-    u64 *dSum = rb.zeros;
-    u32 *idx = rb.idx;
-    u16 *mv = rb.mv;
     Lib &L = lib();
-
-    const QuadBitBoard *pb = b0;
-    const u8 *ps = s0;
-    u64 n = n1, nParents = 1;
     Profiler &P = L.prof;
-    int last = depth - leafPlies;  // level whose entries feed the leaf
-    for (int lvl = 1; lvl < last; lvl++) {
+    const int last = depth - leafPlies;  // level whose entries feed the leaf
+    const QuadBitBoard *pb;
+    const u8 *ps;
+    u32 *idx;
+    u16 *mv;
+    u64 *dSum;  // [leaf sum, leaf chunk queue]
+    u64 n = n1, nParents = 1;
+    int lvl = 1;
+    if (last > 1) {
+        // level 1 from kernel arguments: no upload before the first kernel
+        dSum = A.alloc<u64>(2);
+        QuadBitBoard *ob = A.alloc<QuadBitBoard>(n1);
+        u8 *os = A.alloc<u8>(n1);
+        u32 *nidx = A.alloc<u32>((u64)n1 * MAX_MOVES);
+        u16 *nmv = A.alloc<u16>((u64)n1 * MAX_MOVES);
+        if (!dSum || !ob || !os || !nidx || !nmv) return PG_ERR_ARENA_OOM;
+        RootArgs ra{};
+        ra.board = root;
+        ra.n = n1;
+        ra.state = state;
+        memcpy(ra.moves, rootMoves, sizeof(u16) * n1);
+        volatile u64 *slot = pinned + PIN_POLL;
+        *slot = HOST_SLOT_EMPTY;
+        if (color == WHITE)
+            hipLaunchKernelGGL(k_level_root<WHITE>, dim3(1), dim3(SMALL_LEVEL), 0, st, ra, ob, os, nidx, nmv,
+                               pinned + PIN_POLL, dSum);
+        else
+            hipLaunchKernelGGL(k_level_root<BLACK>, dim3(1), dim3(SMALL_LEVEL), 0, st, ra, ob, os, nidx, nmv,
+                               pinned + PIN_POLL, dSum);
+        PG_HIP(hipGetLastError());
+        u64 c;
+        PG_TRY(waitPublished(slot, st, c));
+        if (c == 0) {
+            PG_HIP(hipStreamSynchronize(st));  // the block may still be running
+            *count = 0;
+            if (peakBytes) *peakBytes = A.peak;
+            P.collect();
+            return PG_OK;
+        }
+        pb = ob;
+        ps = os;
+        idx = nidx;
+        mv = nmv;
+        nParents = n1;
+        n = c;
+        lvl = 2;
+    } else {
+        RootBlock rb;
+        PG_TRY(uploadRoot(A, root, state, rootMoves, n1, 2, pinned, st, rb));
+        pb = rb.b0;
+        ps = rb.s0;
+        dSum = rb.zeros;
+        idx = rb.idx;
+        mv = rb.mv;
+    }
+    for (; lvl < last; lvl++) {
         int pc = color ^ ((lvl - 1) & 1);  // side to move in the parent boards
         if (n <= SMALL_LEVEL) {
             // one block does the whole level; the children's arrays take
