@@ -357,9 +357,9 @@ int gpuBfs(const pg_position *pos, int depth, bool useTT, void *buf, uint64_t by
 // default without the tables.  With them the serial recursion stays the
 // default: each GPU call finds the subtrees the calls before it stored, and
 // that beat both frontier forms on the startpos TT runs (launch depth 9;
-// perft 10 / 11: serial 0.755 / 9.22 s, the frontier over four parallel slots
-// 0.843 / 9.56 s, the frontier through the batched engine with the table
-// 0.897 / 11.91 s).
+// perft 10 / 11: serial 0.685 / 8.47 s, the frontier over four parallel slots
+// 0.770 / 8.77 s, the frontier through the batched engine with the table
+// 0.831 / 11.11 s).
 struct PosKey {
     u64 bb[4];
     u8 state, color;
