@@ -371,7 +371,17 @@ def ref_positions_leg(pg, tt_max_depth: int, tt_only: bool = False) -> dict:
     """The reference README's published positions through the launcher, each
     count checked against the published value: without the tables, then
     startpos with them (64 GB device table, 256 GB host tier, cleared before
-    each run)."""
+    each run).
+
+    Without the tables the launcher's default expands up to four plies above
+    its launch depth on the host, merges identical positions (counted once,
+    times their multiplicity) and runs the rest through the batched engine:
+    `seconds` is that (second call; `seconds_first_call` includes the batched
+    engine's graph captures and tuning trials).  `every_node_seconds` runs the
+    launcher at the memory-based launch depth instead, where the frontier is
+    at most one ply and no position repeats, so every node of the tree is
+    computed as the reference's -nott run computes it."""
+    import ctypes
     known = {p["name"]: p for p in json.load(open(os.path.join(ROOT, "tests", "golden", "known_positions.json")))["positions"]}
     out = {}
     for name, d, theirs in ([] if tt_only else REF_RUNS):
@@ -379,11 +389,23 @@ def ref_positions_leg(pg, tt_max_depth: int, tt_only: bool = False) -> dict:
         want = k["counts"][d - 1]
         p = pg.parse_fen(k["fen"])
         pg.perft(p, d - 2)  # warm (launch depth estimate, first launches)
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            c = pg.perft(p, d)
+            times.append(time.perf_counter() - t0)
+            assert c == want, (name, d, c, want)
+        bf, ld = ctypes.c_float(), ctypes.c_int()
+        pg.lib().pg_estimate_launch_depth(ctypes.byref(p), ctypes.byref(bf), ctypes.byref(ld))
+        pg.perft(p, d - 2, launch_depth=ld.value)
         t0 = time.perf_counter()
-        c = pg.perft(p, d)
-        dt = time.perf_counter() - t0
+        c = pg.perft(p, d, launch_depth=ld.value)
+        dt_all = time.perf_counter() - t0
         assert c == want, (name, d, c, want)
-        out[f"{name}_perft{d}"] = {"nodes": c, "seconds": dt, "nodes_per_s": c / dt, "reference_published": theirs}
+        dt = times[1]
+        out[f"{name}_perft{d}"] = {"nodes": c, "seconds": dt, "nodes_per_s": c / dt, "seconds_first_call": times[0],
+                                   "every_node_seconds": dt_all, "every_node_launch_depth": ld.value,
+                                   "reference_published": theirs}
     pg.tt_init(65536, HOST_TT_MB)
     try:
         for name, d, theirs in REF_RUNS_TT:

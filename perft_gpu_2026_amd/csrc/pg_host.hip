@@ -95,6 +95,7 @@ static bool g_frontierBatch = true;  // launcher: batch the frontier above the l
 enum { TT_FRONTIER_SERIAL = 0, TT_FRONTIER_PARALLEL = 1, TT_FRONTIER_BATCH = 2 };
 static int g_ttFrontier = TT_FRONTIER_SERIAL;
 static const int FRONTIER_MAX_PLIES = 4;
+static const int FRONTIER_AUTO_MIN_DEPTH = 7;  // see perftLaunch
 
 static inline QuadBitBoard toQBB(const pg_position *p)
 {
@@ -605,6 +606,17 @@ int perftLaunch(const pg_position *pos, int depth, int launchDepth, bool useTT, 
     if (launchDepth <= 0) {
         float bf;
         PG_TRY(estimateLD(pos, &bf, &launchDepth));
+        // Without the tables the nodes above the launch depth are expanded on
+        // the host, identical positions merged (with their multiplicities) and
+        // the rest run through the batched engine: a deeper frontier merges
+        // more and leaves less to compute.  From depth 7 the automatic launch
+        // depth leaves a frontier of up to four plies (launch depth
+        // max(4, depth - 4)): startpos perft 9 / 10 took 0.47 / 13.5 s against
+        // 1.18 / 35.3 s at the memory-based launch depth, Kiwipete perft 7 / 8
+        // 0.079 / 2.5 s against 0.122 / 5.8 s, Position 3 perft 10 0.32 s
+        // against 0.74 s; a five-ply frontier at depth 7 was slower.
+        if (!useTT && g_frontierBatch && depth >= FRONTIER_AUTO_MIN_DEPTH)
+            launchDepth = std::min(launchDepth, std::max(4, depth - FRONTIER_MAX_PLIES));
     }
     g_effectiveLD = std::min(launchDepth, depth);
     if (g_effectiveLD < 1) g_effectiveLD = 1;

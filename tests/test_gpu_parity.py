@@ -161,6 +161,19 @@ def test_launcher_host_recursion(gpu, frontier):
         gpu.lib().pg_set_frontier_batch(1)
 
 
+def test_launcher_auto_frontier(gpu):
+    # without the tables the automatic launch depth leaves a frontier of up to
+    # four plies from depth 7 (identical positions merged, then the batched
+    # engine); every count against the published ones
+    cases = [(gpu.START_FEN, 7, 3195901860), (gpu.START_FEN, 8, 84998978956),
+             ("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -", 7, 374190009323),
+             ("8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - -", 8, 3009794393)]
+    for fen, d, want in cases:
+        print(f"[gpu] launcher auto frontier d={d} {fen}", flush=True)
+        assert gpu.perft(gpu.parse_fen(fen), d) == want, (fen, d)
+        assert gpu.lib().pg_get_effective_ld() <= max(4, d - 4), (fen, d)
+
+
 def test_forced_oom_reports_and_falls_back(gpu):
     kiwi = gpu.parse_fen("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -")
     buf = gpu.DeviceArray(64 * 1024)
