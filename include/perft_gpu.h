@@ -122,7 +122,14 @@ int pg_perft_bfs(const pg_position *pos, int depth, int useTT,
 /* perftLauncher, launcher.h:18 / launcher.cu:572-737: host recursion above the
  * launch depth (host TT tier when enabled), GPU BFS at the launch depth, and on
  * PG_ERR_ARENA_OOM a fallback one ply down that lowers the effective launch
- * depth.  launchDepth <= 0 picks it with pg_estimate_launch_depth. */
+ * depth.  launchDepth <= 0 picks it with pg_estimate_launch_depth; then,
+ * without the tables and from depth 7, it is lowered to max(4, depth - 4):
+ * the plies above it are expanded on the host, identical positions merged
+ * (each counted once, times its multiplicity) and the rest run through the
+ * batched engine, which computes fewer nodes than the tree holds.  An explicit
+ * launchDepth is used as given; the perft_gpu command line passes the
+ * reference's launch-depth policy, whose frontier of at most one ply repeats
+ * no position, so every node is computed. */
 int pg_perft(const pg_position *pos, int depth, int launchDepth, int useTT, pg_u128 *count);
 
 /* perft_cpu_dispatch, launcher.h:22-23 / launcher.cu:762-816: the -cpu path,
