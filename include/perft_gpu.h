@@ -123,7 +123,9 @@ int pg_perft_bfs(const pg_position *pos, int depth, int useTT,
  * launch depth (host TT tier when enabled), GPU BFS at the launch depth, and on
  * PG_ERR_ARENA_OOM a fallback one ply down that lowers the effective launch
  * depth.  launchDepth <= 0 picks it with pg_estimate_launch_depth; then,
- * without the tables and from depth 7, it is lowered to max(4, depth - 4):
+ * without the tables and from depth 7, it is lowered to max(4, depth - 4)
+ * (depth - 5 from depth 10; fewer plies where the frontier would pass 8
+ * million positions):
  * the plies above it are expanded on the host, identical positions merged
  * (each counted once, times its multiplicity) and the rest run through the
  * batched engine, which computes fewer nodes than the tree holds.  An explicit

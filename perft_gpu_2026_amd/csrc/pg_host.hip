@@ -94,8 +94,9 @@ static bool g_frontierBatch = true;  // launcher: batch the frontier above the l
 // batched engine with the device table (pg_set_tt_frontier)
 enum { TT_FRONTIER_SERIAL = 0, TT_FRONTIER_PARALLEL = 1, TT_FRONTIER_BATCH = 2 };
 static int g_ttFrontier = TT_FRONTIER_SERIAL;
-static const int FRONTIER_MAX_PLIES = 4;
-static const int FRONTIER_AUTO_MIN_DEPTH = 7;  // see perftLaunch
+static const int FRONTIER_MAX_PLIES = 5;
+static const int FRONTIER_AUTO_MIN_DEPTH = 7;     // see perftLaunch
+static const double FRONTIER_AUTO_MAX_POSITIONS = 8e6;  // bf^plies bound of an automatic frontier
 
 static inline QuadBitBoard toQBB(const pg_position *p)
 {
@@ -610,13 +611,18 @@ int perftLaunch(const pg_position *pos, int depth, int launchDepth, bool useTT, 
         // the host, identical positions merged (with their multiplicities) and
         // the rest run through the batched engine: a deeper frontier merges
         // more and leaves less to compute.  From depth 7 the automatic launch
-        // depth leaves a frontier of up to four plies (launch depth
-        // max(4, depth - 4)): startpos perft 9 / 10 took 0.47 / 13.5 s against
-        // 1.18 / 35.3 s at the memory-based launch depth, Kiwipete perft 7 / 8
-        // 0.079 / 2.5 s against 0.122 / 5.8 s, Position 3 perft 10 0.32 s
-        // against 0.74 s; a five-ply frontier at depth 7 was slower.
-        if (!useTT && g_frontierBatch && depth >= FRONTIER_AUTO_MIN_DEPTH)
-            launchDepth = std::min(launchDepth, std::max(4, depth - FRONTIER_MAX_PLIES));
+        // depth leaves a frontier of four plies, five from depth 10 (launch
+        // depth max(4, depth - plies)), fewer where bf^plies would pass 8
+        // million positions: startpos perft 9 / 10 took 0.47 / 6.5 s against
+        // 1.18 / 35.3 s at the memory-based launch depth (13.5 s for perft 10
+        // with four plies), Kiwipete perft 7 / 8 0.079 / 2.5 s against
+        // 0.122 / 5.8 s, Position 3 perft 10 0.31 s against 0.74 s; a
+        // four-ply frontier at depth 7 was slower than three.
+        if (!useTT && g_frontierBatch && depth >= FRONTIER_AUTO_MIN_DEPTH) {
+            int plies = depth >= 10 ? 5 : 4;
+            while (plies > 1 && pow((double)bf, plies) > FRONTIER_AUTO_MAX_POSITIONS) plies--;
+            launchDepth = std::min(launchDepth, std::max(4, depth - plies));
+        }
     }
     g_effectiveLD = std::min(launchDepth, depth);
     if (g_effectiveLD < 1) g_effectiveLD = 1;
