@@ -380,35 +380,89 @@ struct PosKeyHash {
     }
 };
 
+using PosMap = std::unordered_map<PosKey, u64, PosKeyHash>;
+
+// the children of `parents[b, e)` (with the parents' multiplicities)
+template <class Sink>
+static void expandRange(const std::vector<std::pair<PosKey, u64>> &parents, size_t b, size_t e, Sink &&sink)
+{
+    for (size_t i = b; i < e; i++) {
+        const PosKey &k = parents[i].first;
+        QuadBitBoard q;
+        memcpy(q.bb, k.bb, 32);
+        u16 mv[MAX_MOVES];
+        u32 n = generateMoves(q, k.state, k.color, mv);
+        for (u32 j = 0; j < n; j++) {
+            PosKey c;
+            QuadBitBoard cq = q;
+            c.state = k.state;
+            makeMove(cq, c.state, k.color, mv[j]);
+            memcpy(c.bb, cq.bb, 32);
+            c.color = k.color ^ 1;
+            sink(c, parents[i].second);
+        }
+    }
+}
+
+// One ply of the frontier over the host threads: each thread expands a slice
+// of the parents into per-shard buffers (the shard from the key's hash), then
+// each thread merges one shard.  Startpos's fifth ply (4,865,609 children of
+// 72,078 parents, 822,518 unique) took 0.78 s in one hash map on one thread.
+static std::vector<std::pair<PosKey, u64>> expandPlyParallel(const std::vector<std::pair<PosKey, u64>> &parents)
+{
+    const unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<std::vector<std::pair<PosKey, u64>>>> buf(T, std::vector<std::vector<std::pair<PosKey, u64>>>(T));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++)
+        th.emplace_back([&, t] {
+            const size_t b = parents.size() * t / T, e = parents.size() * (t + 1) / T;
+            expandRange(parents, b, e, [&](const PosKey &c, u64 m) {
+                buf[t][(PosKeyHash()(c) >> 40) % T].emplace_back(c, m);
+            });
+        });
+    for (auto &x : th) x.join();
+    th.clear();
+    std::vector<std::vector<std::pair<PosKey, u64>>> out(T);
+    for (unsigned sh = 0; sh < T; sh++)
+        th.emplace_back([&, sh] {
+            size_t n = 0;
+            for (unsigned t = 0; t < T; t++) n += buf[t][sh].size();
+            PosMap m;
+            m.reserve(n);
+            for (unsigned t = 0; t < T; t++) {
+                for (auto &kv : buf[t][sh]) m[kv.first] += kv.second;
+                std::vector<std::pair<PosKey, u64>>().swap(buf[t][sh]);
+            }
+            out[sh].assign(m.begin(), m.end());
+        });
+    for (auto &x : th) x.join();
+    std::vector<std::pair<PosKey, u64>> all;
+    size_t n = 0;
+    for (auto &o : out) n += o.size();
+    all.reserve(n);
+    for (auto &o : out) all.insert(all.end(), o.begin(), o.end());
+    return all;
+}
+
 // The unique positions `plies` plies below `pos` and how many move sequences
-// reach each (exact: the whole position is the key).
+// reach each (exact: the whole position is the key).  Plies of more than 2^16
+// parents run over the host threads.
 static void expandFrontier(const pg_position *pos, int plies, std::vector<pg_position> &ps, std::vector<u64> &mult)
 {
-    std::unordered_map<PosKey, u64, PosKeyHash> cur, next;
     PosKey k0;
     memcpy(k0.bb, pos->bb, 32);
     k0.state = pos->state;
     k0.color = pos->color;
-    cur[k0] = 1;
+    std::vector<std::pair<PosKey, u64>> cur{{k0, 1}};
     for (int ply = 0; ply < plies; ply++) {
-        next.clear();
-        next.reserve(cur.size() * 32);
-        for (auto &kv : cur) {
-            QuadBitBoard q;
-            memcpy(q.bb, kv.first.bb, 32);
-            u16 mv[MAX_MOVES];
-            u32 n = generateMoves(q, kv.first.state, kv.first.color, mv);
-            for (u32 i = 0; i < n; i++) {
-                PosKey c;
-                QuadBitBoard cq = q;
-                c.state = kv.first.state;
-                makeMove(cq, c.state, kv.first.color, mv[i]);
-                memcpy(c.bb, cq.bb, 32);
-                c.color = kv.first.color ^ 1;
-                next[c] += kv.second;
-            }
+        if (cur.size() > (1u << 16)) {
+            cur = expandPlyParallel(cur);
+            continue;
         }
-        std::swap(cur, next);
+        PosMap next;
+        next.reserve(cur.size() * 32);
+        expandRange(cur, 0, cur.size(), [&](const PosKey &c, u64 m) { next[c] += m; });
+        cur.assign(next.begin(), next.end());
     }
     ps.clear();
     mult.clear();
@@ -613,7 +667,7 @@ int perftLaunch(const pg_position *pos, int depth, int launchDepth, bool useTT, 
         // more and leaves less to compute.  From depth 7 the automatic launch
         // depth leaves a frontier of four plies, five from depth 10 (launch
         // depth max(4, depth - plies)), fewer where bf^plies would pass 8
-        // million positions: startpos perft 9 / 10 took 0.47 / 6.5 s against
+        // million positions: startpos perft 9 / 10 took 0.47 / 6.35 s against
         // 1.18 / 35.3 s at the memory-based launch depth (13.5 s for perft 10
         // with four plies), Kiwipete perft 7 / 8 0.079 / 2.5 s against
         // 0.122 / 5.8 s, Position 3 perft 10 0.31 s against 0.74 s; a
