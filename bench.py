@@ -38,7 +38,10 @@ against golden counts: the level kernels at full size (above); the
 4096-position config-3 batch, batched vs plain; config 4 (64 positions,
 perft 8) with the transposition tables, reporting table probes/s; the
 reference README's published positions (startpos perft 9, Kiwipete perft 7,
-Position 3 perft 10 without the tables; startpos perft 9-11 with them) with the
+Position 3 perft 10 without the tables, each at the launcher's automatic
+launch depth, which merges repeated positions of a host-expanded frontier,
+and at the memory-based launch depth, which computes every node as the
+reference's -nott run does; startpos perft 9-11 with the tables) with the
 published times beside them; and the config-5 serving loop, replaying N requests
 (default all 10,000) with Poisson arrivals (rate R requests/s; default 0.75 x
 the capacity measured by a closed-loop pass), reporting p50/p99 latency and
