@@ -174,6 +174,14 @@ def test_launcher_auto_frontier(gpu):
         assert gpu.lib().pg_get_effective_ld() <= max(4, d - 4), (fen, d)
 
 
+def test_launcher_frontier_over_host_threads(gpu):
+    # a five-ply frontier from the start position: its last ply (72,078
+    # parents, 4,865,609 children, 822,518 unique) is expanded over the host
+    # threads into hashed shards
+    print("[gpu] five-ply frontier, startpos perft 8 at launch depth 3", flush=True)
+    assert gpu.perft(gpu.parse_fen(gpu.START_FEN), 8, launch_depth=3) == 84998978956
+
+
 def test_forced_oom_reports_and_falls_back(gpu):
     kiwi = gpu.parse_fen("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq -")
     buf = gpu.DeviceArray(64 * 1024)
